@@ -1,0 +1,255 @@
+"""grus_framework_amd -- MI355X (gfx950) graph-traversal engine.
+
+Python front-end over the C-ABI of libgrus_hip.so (include/grus_c.h).  The
+engine itself is C++/HIP; this module only moves numpy arrays across the
+boundary.  Names follow the reference's surface: graph loading
+(graph_loader::Load, include/graph_loader.cuh:160), placement
+(Set_Mem_Policy, include/graph.cuh:170), and the BFS / SSSP / PageRank / CC
+apps (samples/*).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import (DIR_AUTO, DIR_PULL, DIR_PUSH, INFINIT, PLACE_CAPPED, PLACE_HBM,
+                   PLACE_MANAGED, Q_BFS, Q_SSSP, GrusError, Place, Stats, Tuning)
+
+__all__ = ["Context", "Graph", "Server", "bfs", "sssp", "pagerank", "cc", "bfs_multi",
+           "sssp_multi", "gr_header", "gr_read", "gr_write", "device_count", "GrusError",
+           "INFINIT", "Place", "Stats", "Tuning", "place"]
+
+_DIRS = {"auto": DIR_AUTO, "push": DIR_PUSH, "pull": DIR_PULL}
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _lib():
+    return capi.lib()
+
+
+def device_count():
+    n = C.c_int(0)
+    capi.check(_lib().grus_device_count(C.byref(n)))
+    return n.value
+
+
+def place(mode="hbm", hbm_cap_bytes=0, pf=False, ab=False, rm=False, pl=False):
+    """Placement request; flag names follow include/app.cuh:27-31."""
+    modes = {"hbm": PLACE_HBM, "managed": PLACE_MANAGED, "capped": PLACE_CAPPED}
+    return Place(modes[mode], int(hbm_cap_bytes), int(pf), int(ab), int(rm), int(pl))
+
+
+# ------------------------------------------------------------------ .gr I/O
+def gr_header(path):
+    V, E, wt = C.c_uint64(), C.c_uint64(), C.c_int()
+    capi.check(_lib().grus_gr_header(path.encode(), C.byref(V), C.byref(E), C.byref(wt)))
+    return V.value, E.value, bool(wt.value)
+
+
+def gr_read(path, need_weight=True):
+    V, E, _ = gr_header(path)
+    off = np.zeros(V + 1, np.uint64)
+    col = np.zeros(max(E, 1), np.uint32)
+    w = np.zeros(max(E, 1), np.uint32) if need_weight else None
+    capi.check(_lib().grus_gr_read(path.encode(), _p(off), _p(col), _p(w)))
+    return off, col[:E], (None if w is None else w[:E])
+
+
+def gr_write(path, off, col, w=None):
+    off = np.ascontiguousarray(off, np.uint64)
+    col = np.ascontiguousarray(col, np.uint32)
+    w = None if w is None else np.ascontiguousarray(w, np.uint32)
+    capi.check(_lib().grus_gr_write(path.encode(), len(off) - 1, len(col), _p(off), _p(col),
+                                     _p(w)))
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One device + one HIP stream (include/grus_c.h grus_init)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        capi.check(_lib().grus_init(device, C.byref(h)))
+        self.h = h
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        capi.check(_lib().grus_get_tuning(self.h, C.byref(t)))
+        return t
+
+    @tuning.setter
+    def tuning(self, t):
+        capi.check(_lib().grus_set_tuning(self.h, C.byref(t)))
+
+    def close(self):
+        if self.h:
+            capi.check(_lib().grus_fini(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Graph:
+    """Device-resident CSR graph (graph_t<CSR>, include/graph.cuh:254)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        info = capi.GraphInfo()
+        capi.check(_lib().grus_graph_info_get(self.h, C.byref(info)))
+        self.info = info
+        self.V = info.V
+        self.E = info.E
+
+    @classmethod
+    def from_csr(cls, ctx, off, col, w=None, symmetric=False, placement=None):
+        off = np.ascontiguousarray(off, np.uint64)
+        col = np.ascontiguousarray(col, np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, np.uint32)
+        h = C.c_void_p()
+        pl = placement if placement is not None else place()
+        capi.check(_lib().grus_graph_from_csr(ctx.h, len(off) - 1, len(col), _p(off), _p(col),
+                                              _p(w), int(symmetric), C.byref(pl), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load_gr(cls, ctx, path, need_weight=True, symmetric=False, placement=None):
+        h = C.c_void_p()
+        pl = placement if placement is not None else place()
+        capi.check(_lib().grus_graph_load_gr(ctx.h, path.encode(), int(need_weight),
+                                             int(symmetric), C.byref(pl), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def rmat(cls, ctx, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weights=None,
+             wseed=0, placement=None):
+        wlo, whi = weights if weights else (0, 0)
+        h = C.c_void_p()
+        pl = placement if placement is not None else place()
+        capi.check(_lib().grus_graph_gen_rmat(ctx.h, scale, edge_factor, a, b, c, seed, wlo, whi,
+                                              wseed, C.byref(pl), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def grid(cls, ctx, rows, cols, weights=None, wseed=0, placement=None):
+        wlo, whi = weights if weights else (0, 0)
+        h = C.c_void_p()
+        pl = placement if placement is not None else place()
+        capi.check(_lib().grus_graph_gen_grid(ctx.h, rows, cols, wlo, whi, wseed, C.byref(pl),
+                                              C.byref(h)))
+        return cls(ctx, h)
+
+    def csr(self):
+        off = np.zeros(self.V + 1, np.uint64)
+        col = np.zeros(max(self.E, 1), np.uint32)
+        w = np.zeros(max(self.E, 1), np.uint32) if self.info.weighted else None
+        capi.check(_lib().grus_graph_copy_csr(self.h, _p(off), _p(col), _p(w)))
+        return off, col[:self.E], (None if w is None else w[:self.E])
+
+    def result(self, dtype=np.uint32):
+        out = np.zeros(self.V, dtype)
+        capi.check(_lib().grus_result_copy(self.h, _p(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            capi.check(_lib().grus_graph_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+# ------------------------------------------------------------------ apps
+def bfs(g, src, direction="auto", copy=True):
+    """BFS levels (samples/bfs/bfs.cu); returns (levels or None, Stats)."""
+    st = Stats()
+    out = np.zeros(g.V, np.uint32) if copy else None
+    capi.check(_lib().grus_bfs(g.h, src, _DIRS[direction], _p(out), C.byref(st)))
+    return out, st
+
+
+def sssp(g, src, copy=True):
+    """SSSP distances (samples/sssp/sssp.cu); returns (dist or None, Stats)."""
+    st = Stats()
+    out = np.zeros(g.V, np.uint32) if copy else None
+    capi.check(_lib().grus_sssp(g.h, src, _p(out), C.byref(st)))
+    return out, st
+
+
+def pagerank(g, alpha=0.85, iters=20, copy=True):
+    """FP32 pull PageRank (samples/pagerank/pr.cu, fixed iterations)."""
+    st = Stats()
+    out = np.zeros(g.V, np.float32) if copy else None
+    capi.check(_lib().grus_pagerank(g.h, alpha, iters, _p(out), C.byref(st)))
+    return out, st
+
+
+def cc(g, method="hook", copy=True):
+    """Component labels = min vertex id (samples/cc/cc.cu)."""
+    st = Stats()
+    out = np.zeros(g.V, np.uint32) if copy else None
+    m = {"hook": 0, "lp": 1}[method]
+    capi.check(_lib().grus_cc(g.h, m, _p(out), C.byref(st)))
+    return out, st
+
+
+def bfs_multi(g, srcs, copy=True):
+    srcs = np.ascontiguousarray(srcs, np.uint32)
+    st = Stats()
+    out = np.zeros((len(srcs), g.V), np.uint32) if copy else None
+    capi.check(_lib().grus_bfs_multi(g.h, _p(srcs), len(srcs), _p(out), C.byref(st)))
+    return out, st
+
+
+def sssp_multi(g, srcs, copy=True):
+    srcs = np.ascontiguousarray(srcs, np.uint32)
+    st = Stats()
+    out = np.zeros((len(srcs), g.V), np.uint32) if copy else None
+    capi.check(_lib().grus_sssp_multi(g.h, _p(srcs), len(srcs), _p(out), C.byref(st)))
+    return out, st
+
+
+class Server:
+    """Continuous-batching query server (grus_server_*)."""
+
+    def __init__(self, g, max_inflight_batches=2, max_batch=64, state_budget_bytes=0,
+                 copy_results=False):
+        self.g = g
+        opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
+                               int(copy_results))
+        h = C.c_void_p()
+        capi.check(_lib().grus_server_create(g.h, C.byref(opts), C.byref(h)))
+        self.h = h
+
+    def submit(self, kind, src, out=None):
+        qid = C.c_uint64()
+        capi.check(_lib().grus_server_submit(self.h, kind, src, _p(out), C.byref(qid)))
+        return qid.value
+
+    def poll(self, timeout_ms=-1):
+        r = capi.QueryResult()
+        capi.check(_lib().grus_server_poll(self.h, C.byref(r), timeout_ms))
+        return r
+
+    def close(self):
+        if self.h:
+            capi.check(_lib().grus_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,122 @@
+// advance.hip -- frontier conversion helpers and the push operator's scan.
+//
+// bitmap_to_queue replaces flag_to_wl (reference src/worklist.cu:87-93,
+// one aggregated atomic per 32-lane group over a byte flag per vertex):
+// here one lane owns a 64-vertex bitmap word, a wave prefix-sums the
+// popcounts and one atomic per wave reserves the output range, so the
+// output keeps ascending order inside each wave.  queue_to_bitmap replaces
+// wl_to_flag (src/worklist.cu:94-100) with bit-level atomics.
+#include <algorithm>
+
+#include "advance.hpp"
+#include "prims.hpp"
+
+namespace mgg {
+
+namespace {
+
+__global__ void k_queue_degrees(const uint64_t *off, const uint32_t *q, uint32_t F,
+                                uint64_t *deg) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i <= F; i += stride) {
+    uint64_t d = 0;
+    if (i < F) {
+      uint32_t u = q[i];
+      d = off[u + 1] - off[u];
+    }
+    deg[i] = d;
+  }
+}
+
+__global__ void k_queue_to_bitmap(const uint32_t *q, uint32_t n, uint64_t *bm) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint32_t v = q[i];
+    atomicOr((unsigned long long *)&bm[v >> 6], 1ull << (v & 63));
+  }
+}
+
+__global__ void k_bitmap_to_queue(const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // uniform trip count per wave: iterate over wave-aligned bases
+  uint64_t first = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~(uint64_t)(WAVE - 1));
+  for (uint64_t base = first; base < W; base += stride) {
+    uint64_t wd = base + lane_id();
+    uint64_t m = wd < W ? bm[wd] : 0ull;
+    uint32_t c = (uint32_t)__popcll(m);
+    uint32_t x = c;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      uint32_t y = __shfl_up(x, o);
+      if ((int)lane_id() >= o) x += y;
+    }
+    uint32_t total = __shfl(x, WAVE - 1);
+    uint32_t at = 0;
+    if (lane_id() == 0 && total) at = atomicAdd(count, total);
+    at = __shfl(at, 0) + (x - c);
+    while (m) {
+      int b = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      q[at++] = (uint32_t)(wd * 64 + b);
+    }
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *a, uint64_t n, uint32_t v) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) a[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t *a, uint64_t n, uint64_t v) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) a[i] = v;
+}
+
+}  // namespace
+
+unsigned grid_cap(Context &ctx, uint64_t items, unsigned per_block, unsigned blocks_per_cu) {
+  uint64_t g = div_up(items ? items : 1, per_block);
+  uint64_t cap = (uint64_t)ctx.cu_count * blocks_per_cu;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+unsigned push_grid(Context &ctx, uint64_t total) { return grid_cap(ctx, total, LB_CHUNK, 8); }
+
+void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uint32_t *q,
+                       uint32_t F) {
+  k_queue_degrees<<<grid_cap(ctx, (uint64_t)F + 1, BLOCK), BLOCK, 0, ctx.stream>>>(
+      off, q, F, w.deg.as<uint64_t>());
+  GRUS_LAUNCH_CHECK();
+  scan_excl_u64(ctx, w.deg.as<uint64_t>(), w.scan.as<uint64_t>(), (size_t)F + 1);
+}
+
+void bitmap_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count) {
+  k_bitmap_to_queue<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(bm, W, q, count);
+  GRUS_LAUNCH_CHECK();
+}
+
+void queue_to_bitmap(Context &ctx, const uint32_t *q, uint32_t n, uint64_t *bm, uint64_t W) {
+  fill_u64(ctx, bm, W, 0);
+  if (n) {
+    k_queue_to_bitmap<<<grid_cap(ctx, n, BLOCK), BLOCK, 0, ctx.stream>>>(q, n, bm);
+    GRUS_LAUNCH_CHECK();
+  }
+}
+
+void fill_u32(Context &ctx, uint32_t *a, uint64_t n, uint32_t v) {
+  if (!n) return;
+  k_fill_u32<<<grid_cap(ctx, n, BLOCK), BLOCK, 0, ctx.stream>>>(a, n, v);
+  GRUS_LAUNCH_CHECK();
+}
+
+void fill_u64(Context &ctx, uint64_t *a, uint64_t n, uint64_t v) {
+  if (!n) return;
+  k_fill_u64<<<grid_cap(ctx, n, BLOCK), BLOCK, 0, ctx.stream>>>(a, n, v);
+  GRUS_LAUNCH_CHECK();
+}
+
+}  // namespace mgg

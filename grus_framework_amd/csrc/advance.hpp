@@ -1,0 +1,163 @@
+// advance.hpp -- frontier operators shared by the apps.
+//
+// Reference counterparts: push_kernel (include/kernel.cuh:42-123),
+// flag_to_wl / wl_to_flag (src/worklist.cu:87-100) and the Frontier<BDF>
+// (include/frontier.cuh:27-76).  The reference expands one 32-lane warp per
+// frontier vertex, so a hub of degree 10^5 serializes in one warp and
+// vertices of degree < 32 idle most lanes.  Here the push operator is
+// edge-balanced: the frontier's out-degrees are prefix-summed, every
+// workgroup takes a fixed-size chunk of the frontier's edge space, stages
+// the owning queue entries in LDS and assigns one edge per lane (owner
+// found by binary search in LDS), so any degree mix keeps all 64 lanes of
+// every wave busy and all column reads coalesced.
+//
+// Functor contract kept from the reference (samples/bfs/bfs.cu:51-87):
+//   bool Job::update(src, dst, edge_id, weight)  -- the app's updater
+//   bool Job::generate(updated, dst)             -- the app's generator:
+//        true when dst joins the next frontier (deduplicated)
+// The operator appends generated vertices to the next queue with one
+// atomic per wave and counts the next frontier's size and out-edges (the
+// inputs of the push/pull direction heuristic).
+#pragma once
+
+#include "device.hpp"
+#include "graph.hpp"
+
+namespace mgg {
+
+// counters of one frontier step, in the context's device control block
+struct StepCounters {
+  uint32_t next_count;   // vertices appended to the next queue
+  uint32_t long_count;   // bottom-up vertices handed to wave-cooperative scan
+  uint64_t next_edges;   // sum of out-degrees of the next frontier
+  uint64_t edges_seen;   // edges examined (diagnostics)
+  uint64_t pad[5];
+};
+
+__device__ __forceinline__ uint32_t edge_col(const CsrView &g, uint64_t e) {
+  return e < g.split ? g.col[e] : g.col_far[e - g.split];
+}
+__device__ __forceinline__ uint32_t edge_w(const CsrView &g, uint64_t e) {
+  if (!g.w) return 1u;
+  return e < g.split ? g.w[e] : g.w_far[e - g.split];
+}
+
+// ---------------------------------------------------------------- helpers
+// (host wrappers; kernels live in advance.hip)
+// q <- every set bit of bm[0..W) in ascending order within each workgroup;
+// *count (device) receives the number appended (must be zero on entry)
+void bitmap_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count);
+// bm[0..W) <- bitmap of q[0..n)
+void queue_to_bitmap(Context &ctx, const uint32_t *q, uint32_t n, uint64_t *bm, uint64_t W);
+void fill_u32(Context &ctx, uint32_t *a, uint64_t n, uint32_t v);
+void fill_u64(Context &ctx, uint64_t *a, uint64_t n, uint64_t v);
+unsigned grid_cap(Context &ctx, uint64_t items, unsigned per_block, unsigned blocks_per_cu = 16);
+
+constexpr int LB_CHUNK = 1024;          // edges per workgroup step (4 per lane)
+constexpr int LB_WIN = LB_CHUNK + 64;   // queue entries staged in LDS
+
+// first index in [lo, hi) with a[idx] > key (a non-decreasing), hi if none
+__device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t lo, uint32_t hi,
+                                                    uint64_t key) {
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo) >> 1);
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Edge-balanced push over queue q[0..F) with S = exclusive scan of its
+// out-degrees (S[F] = total).  Job supplies update/generate/next-queue.
+template <class Job>
+__global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
+                                                   const uint64_t *S, uint32_t F, uint64_t total,
+                                                   Job job, uint32_t *qn, StepCounters *cnt) {
+  __shared__ uint64_t sS[LB_WIN];
+  __shared__ uint64_t sRow[LB_WIN];
+  __shared__ uint32_t sQ[LB_WIN];
+  __shared__ uint32_t s_i0, s_n;
+  uint64_t my_edges = 0;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * LB_CHUNK; c0 < total;
+       c0 += (uint64_t)gridDim.x * LB_CHUNK) {
+    uint64_t c1 = c0 + LB_CHUNK < total ? c0 + LB_CHUNK : total;
+    if (threadIdx.x == 0) {
+      uint32_t i0 = upper_bound_u64(S, 0, F + 1, c0) - 1;
+      uint32_t i1 = upper_bound_u64(S, i0, F + 1, c1 - 1) - 1;
+      s_i0 = i0;
+      s_n = i1 - i0 + 2;
+    }
+    __syncthreads();
+    const uint32_t i0 = s_i0, n = s_n;
+    const bool staged = n <= LB_WIN;
+    if (staged) {
+      for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        sS[t] = S[i0 + t];
+        if (i0 + t < F) {
+          uint32_t u = q[i0 + t];
+          sQ[t] = u;
+          sRow[t] = g.off[u];
+        }
+      }
+    }
+    __syncthreads();
+    for (uint64_t kb = c0; kb < c1; kb += blockDim.x) {
+      uint64_t k = kb + threadIdx.x;
+      bool valid = k < c1;
+      bool gen = false;
+      uint32_t dst = 0;
+      if (valid) {
+        uint32_t j, u;
+        uint64_t base, row;
+        if (staged) {
+          j = upper_bound_u64(sS, 0, n, k) - 1;
+          base = sS[j];
+          u = sQ[j];
+          row = sRow[j];
+        } else {
+          j = upper_bound_u64(S + i0, 0, n, k) - 1;
+          base = S[i0 + j];
+          u = q[i0 + j];
+          row = g.off[u];
+        }
+        uint64_t e = row + (k - base);
+        dst = edge_col(g, e);
+        bool upd = job.update(u, dst, e, edge_w(g, e));
+        gen = job.generate(upd, dst);
+      }
+      uint32_t slot = wave_append(&cnt->next_count, gen);
+      if (gen) {
+        qn[slot] = dst;
+        my_edges += job.out_degree(dst);
+      }
+    }
+    __syncthreads();
+  }
+  my_edges = wave_sum(my_edges);
+  if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+}
+
+// Host side of the push operator: degree scan + edge-balanced expansion.
+struct PushWork {
+  Buf deg, scan;
+  void ensure(uint64_t V) {
+    deg.ensure((V + 2) * 8);
+    scan.ensure((V + 2) * 8);
+  }
+};
+
+// w.scan[0..F] <- exclusive scan of outdeg(q[i]); w.scan[F] = total
+void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uint32_t *q,
+                       uint32_t F);
+unsigned push_grid(Context &ctx, uint64_t total);
+
+template <class Job>
+void push_advance(Context &ctx, PushWork &w, const CsrView &g, const uint32_t *q, uint32_t F,
+                  uint64_t total, const Job &job, uint32_t *qn, StepCounters *cnt) {
+  if (F == 0 || total == 0) return;
+  k_push_lb<Job><<<push_grid(ctx, total), BLOCK, 0, ctx.stream>>>(
+      g, q, w.scan.as<uint64_t>(), F, total, job, qn, cnt);
+  GRUS_LAUNCH_CHECK();
+}
+
+}  // namespace mgg

@@ -1,0 +1,294 @@
+// bfs.hip -- breadth-first search levels, direction-optimizing.
+//
+// Reference: samples/bfs/bfs.cu (BFSInit :18-23, job_t :25-49, updater
+// :51-60, generator :61-79, pull_selector :80-87, push driver :135-173,
+// pull driver :104-134) on kernel.cuh's push_kernel / pull_kernel.
+//
+// Semantics kept: level[src] = 0, level[v] = iteration + 1 at first
+// discovery, unreached = INFINIT.  Levels are unique, so the result is
+// bit-exact whatever the discovery order.
+//
+// MI355X design:
+//   top-down  (push): edge-balanced LB operator (advance.hpp).  The updater
+//     claims dst with an atomicOr on a visited bitmap (V/8 bytes, L2
+//     resident) instead of the reference's non-atomic label test, so each
+//     vertex is written once and enters the next queue exactly once.
+//   bottom-up (pull): one wave per 64-vertex bitmap word; a lane owns a
+//     vertex that is still unvisited (the reference's pull_selector
+//     label==INFINIT, read from the visited bitmap) and scans its in-edges
+//     for a parent in the current frontier bitmap, stopping at the first
+//     hit (the reference scans every in-edge, kernel.cuh:31-37).  After
+//     `bu_thread_edges` edges a vertex is handed to a wave-cooperative scan
+//     (64 edges per step, ballot early exit).  The wave ballots its found
+//     lanes and writes the next frontier word and visited word with plain
+//     stores -- no atomics -- and its level stores are coalesced.
+//   direction switch (Beamer et al., SC'12): push->pull when the
+//     frontier's out-edges m_f exceed m_u/alpha (m_u = edges of unvisited
+//     vertices), pull->push when the frontier has shrunk below V/beta.
+//     Replaces Frontier<BDF_AUTO>'s fraction thresholds
+//     (include/frontier.cuh:140,206-215).
+#include <algorithm>
+
+#include "advance.hpp"
+#include "apps.hpp"
+
+namespace mgg {
+
+struct BfsWork {
+  Buf visited, front[2], q[2], longq;
+  PushWork push;
+  uint64_t cap_v = 0;
+  void ensure(uint64_t V) {
+    if (V <= cap_v) return;
+    uint64_t W = (V + 63) / 64;
+    visited.alloc((W + 1) * 8);
+    front[0].alloc((W + 1) * 8);
+    front[1].alloc((W + 1) * 8);
+    q[0].alloc((V + 1) * 4);
+    q[1].alloc((V + 1) * 4);
+    longq.alloc((V + 1) * 4);
+    push.ensure(V);
+    cap_v = V;
+  }
+};
+
+namespace bfs {
+
+// job_t of samples/bfs/bfs.cu:25-49 as the device-side job of one level
+struct job_t {
+  uint32_t *label;
+  uint64_t *visited;
+  const uint64_t *out_off;
+  uint32_t next_level;  // job.itr + 1
+  // updater (bfs.cu:51-60): first discovery sets the level; the bitmap
+  // claim makes the check-then-store race-free
+  __device__ __forceinline__ bool update(vtx_t, vtx_t dst, eid_t, weight_t) const {
+    if (bm_test(visited, dst)) return false;
+    if (!bm_claim(visited, dst)) return false;
+    label[dst] = next_level;
+    return true;
+  }
+  // generator (bfs.cu:67-71): a claimed vertex joins the next frontier once
+  __device__ __forceinline__ bool generate(bool updated, vtx_t) const { return updated; }
+  __device__ __forceinline__ uint64_t out_degree(vtx_t v) const {
+    return out_off[v + 1] - out_off[v];
+  }
+};
+
+__global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uint64_t *unreachable,
+                       uint64_t W, uint32_t src, uint32_t *q0) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t n = V > W ? V : W;
+  for (; i < n; i += stride) {
+    if (i < V) label[i] = (i == src) ? 0u : INFINIT;  // BFSInit bfs.cu:21
+    if (i < W) {
+      uint64_t m = unreachable ? unreachable[i] : 0ull;
+      if (i == (src >> 6)) m |= 1ull << (src & 63);
+      visited[i] = m;
+    }
+    if (i == 0) q0[0] = src;
+  }
+}
+
+// bottom-up step over every bitmap word (see file comment)
+__global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_off,
+                                                const uint64_t *front, uint64_t *visited,
+                                                uint64_t *front_next, uint32_t *label,
+                                                uint32_t next_level, uint32_t K, uint32_t *longq,
+                                                StepCounters *cnt, uint64_t W) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  uint32_t lane = lane_id();
+  uint32_t my_found = 0;
+  uint64_t my_edges = 0;
+  for (uint64_t wd = wave; wd < W; wd += nwaves) {
+    uint64_t vis = visited[wd];
+    if (vis == ~0ull) {
+      if (lane == 0) front_next[wd] = 0;
+      continue;
+    }
+    uint32_t v = (uint32_t)(wd * 64 + lane);
+    bool active = ((vis >> lane) & 1ull) == 0;  // pull_selector: label == INFINIT
+    bool found = false, is_long = false;
+    if (active) {
+      uint64_t e = in.off[v], end = in.off[v + 1];
+      uint64_t lim = (end - e > K) ? e + K : end;
+      for (; e < lim; e++) {
+        if (bm_test(front, edge_col(in, e))) {
+          found = true;
+          break;
+        }
+      }
+      is_long = !found && lim < end;
+    }
+    uint64_t m = __ballot(found);
+    if (lane == 0) {
+      front_next[wd] = m;
+      if (m) visited[wd] = vis | m;
+    }
+    if (found) {
+      label[v] = next_level;
+      my_found++;
+      my_edges += out_off[v + 1] - out_off[v];
+    }
+    uint32_t slot = wave_append(&cnt->long_count, is_long);
+    if (is_long) longq[slot] = v;
+  }
+  my_found = wave_sum(my_found);
+  my_edges = wave_sum(my_edges);
+  if (lane == 0 && my_found) {
+    atomicAdd(&cnt->next_count, my_found);
+    atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+  }
+}
+
+// wave-cooperative scan of the vertices whose first K in-edges held no parent
+__global__ __launch_bounds__(BLOCK) void k_pull_long(CsrView in, const uint64_t *out_off,
+                                                     const uint64_t *front, uint64_t *visited,
+                                                     uint64_t *front_next, uint32_t *label,
+                                                     uint32_t next_level, uint32_t K,
+                                                     const uint32_t *longq, StepCounters *cnt) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  uint32_t lane = lane_id();
+  const uint32_t n = cnt->long_count;
+  uint32_t my_found = 0;
+  uint64_t my_edges = 0;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    uint32_t v = longq[i];
+    uint64_t end = in.off[v + 1];
+    bool found = false;
+    for (uint64_t base = in.off[v] + K; base < end; base += WAVE) {
+      uint64_t e = base + lane;
+      bool hit = e < end && bm_test(front, edge_col(in, e));
+      if (__ballot(hit)) {
+        found = true;
+        break;
+      }
+    }
+    if (found && lane == 0) {
+      label[v] = next_level;
+      unsigned long long bit = 1ull << (v & 63);
+      atomicOr((unsigned long long *)&front_next[v >> 6], bit);
+      atomicOr((unsigned long long *)&visited[v >> 6], bit);
+      my_found++;
+      my_edges += out_off[v + 1] - out_off[v];
+    }
+  }
+  if (lane == 0 && my_found) {
+    atomicAdd(&cnt->next_count, my_found);
+    atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+  }
+}
+
+}  // namespace bfs
+
+void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
+  Context &ctx = *g.ctx;
+  ctx.activate();
+  if (src >= g.V) raise(GRUS_E_INVALID, "source vertex out of range");
+  if (dir < GRUS_DIR_AUTO || dir > GRUS_DIR_PULL) raise(GRUS_E_INVALID, "direction mode");
+  if (!g.bfs) g.bfs = std::make_shared<BfsWork>();
+  BfsWork &w = *g.bfs;
+  w.ensure(g.V);
+  g.result.ensure(g.V * 4);
+  const uint64_t V = g.V, W = g.words();
+  const CsrView out = g.csr();
+  const CsrView in = (dir == GRUS_DIR_PUSH) ? out : g.csc();
+  // push-only runs need no in-degrees (and so no CSC of a directed graph)
+  const uint64_t *unreach = (dir == GRUS_DIR_PUSH) ? nullptr : g.unreachable();
+  uint32_t *label = g.result.as<uint32_t>();
+  auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
+  const double alpha = ctx.tuning.bfs_alpha > 0 ? ctx.tuning.bfs_alpha : 14.0;
+  const double beta = ctx.tuning.bfs_beta > 0 ? ctx.tuning.bfs_beta : 24.0;
+  const uint32_t K = ctx.tuning.bu_thread_edges ? ctx.tuning.bu_thread_edges : 32;
+
+  uint64_t src_deg = 0;
+  {
+    uint64_t o[2];
+    GRUS_HIP(hipMemcpy(o, g.off.as<uint64_t>() + src, 16, hipMemcpyDeviceToHost));
+    src_deg = o[1] - o[0];
+  }
+  uint64_t launches = 0;
+  GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  bfs::k_init<<<grid_cap(ctx, std::max(V, W), BLOCK), BLOCK, 0, ctx.stream>>>(
+      label, V, w.visited.as<uint64_t>(), unreach, W, src, w.q[0].as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  launches++;
+
+  int cur = 0;               // index of the current queue / bitmap
+  bool have_queue = true;    // current frontier available as a queue
+  bool have_bitmap = false;  // ... as a bitmap
+  bool pull = (dir == GRUS_DIR_PULL);
+  uint64_t nf = 1, mf = src_deg, prev_nf = 0;
+  uint64_t mu = g.E >= src_deg ? g.E - src_deg : 0;
+  uint64_t v_reached = 1, e_reached = src_deg;
+  uint32_t level = 0, pull_levels = 0;
+  while (nf) {
+    // every level discovers at least one new vertex, so V levels is a bound
+    if (level > V) raise(GRUS_E_HIP, "bfs did not converge within V levels");
+    if (dir == GRUS_DIR_AUTO) {
+      if (!pull && (double)mf > (double)mu / alpha) pull = true;
+      else if (pull && (double)nf < (double)V / beta && nf < prev_nf) pull = false;
+    }
+    GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+    if (pull) {
+      if (!have_bitmap) {
+        queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nf, w.front[cur].as<uint64_t>(), W);
+        launches += 2;
+      }
+      bfs::k_pull<<<grid_cap(ctx, W * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+          in, out.off, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),
+          w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K, w.longq.as<uint32_t>(), cnt, W);
+      GRUS_LAUNCH_CHECK();
+      bfs::k_pull_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+          in, out.off, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),
+          w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K, w.longq.as<uint32_t>(), cnt);
+      GRUS_LAUNCH_CHECK();
+      launches += 2;
+      have_bitmap = true;
+      have_queue = false;
+      pull_levels++;
+    } else {
+      if (!have_queue) {
+        bitmap_to_queue(ctx, w.front[cur].as<uint64_t>(), W, w.q[cur].as<uint32_t>(),
+                        &cnt->long_count);  // scratch counter, re-zeroed below
+        GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+        launches += 1;
+      }
+      push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nf);
+      bfs::job_t job{label, w.visited.as<uint64_t>(), out.off, level + 1};
+      push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, job,
+                   w.q[cur ^ 1].as<uint32_t>(), cnt);
+      launches += 3;
+      have_queue = true;
+      have_bitmap = false;
+    }
+    ctx.read_ctl(2);  // next_count | long_count, next_edges
+    const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
+    prev_nf = nf;
+    nf = hc->next_count;
+    mf = hc->next_edges;
+    mu = mu >= mf ? mu - mf : 0;
+    v_reached += nf;
+    e_reached += mf;
+    cur ^= 1;
+    level++;
+  }
+  GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  GRUS_HIP(hipEventSynchronize(ctx.ev1));
+  float ms = 0;
+  GRUS_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+  if (st) {
+    *st = grus_stats{};
+    st->device_ms = ms;
+    st->iterations = level;
+    st->pull_iterations = pull_levels;
+    st->vertices_reached = v_reached;
+    st->edges_reached = e_reached;
+    st->kernel_launches = launches;
+  }
+}
+
+}  // namespace mgg

@@ -1,0 +1,61 @@
+// device.hpp -- wave64 intrinsics and bitmap frontier primitives for gfx950.
+//
+// atomicAggInc (reference include/intrinsics.cuh:96-103) aggregates one
+// atomicAdd per 32-lane coalesced group through cooperative_groups; here the
+// same slot allocation is a 64-lane ballot + popcount + mbcnt, one atomic
+// per wave, with the leader's base broadcast by a lane shuffle.
+#pragma once
+
+#include "common.hpp"
+
+namespace mgg {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// number of set bits of m in lanes below this one
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Wave-aggregated append: every active lane of the wave must call it.
+// Returns the slot for lanes with pred set, UINT32_MAX otherwise.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *count, bool pred) {
+  uint64_t m = __ballot(pred);
+  if (m == 0) return 0xFFFFFFFFu;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return pred ? base + lane_rank(m) : 0xFFFFFFFFu;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__device__ __forceinline__ bool bm_test(const uint64_t *bm, uint32_t v) {
+  return (bm[v >> 6] >> (v & 63)) & 1ull;
+}
+
+// Sets bit v; true when this call flipped it from 0 to 1.
+__device__ __forceinline__ bool bm_claim(uint64_t *bm, uint32_t v) {
+  unsigned long long bit = 1ull << (v & 63);
+  unsigned long long old = atomicOr((unsigned long long *)&bm[v >> 6], bit);
+  return (old & bit) == 0;
+}
+
+__device__ __forceinline__ uint32_t atomic_min_u32(uint32_t *p, uint32_t v) {
+  return atomicMin(p, v);
+}
+
+// 64-bit splitmix64 output function (Steele, Lea, Flood 2014).
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace mgg

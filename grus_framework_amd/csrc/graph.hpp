@@ -1,0 +1,149 @@
+// graph.hpp -- context, device buffers and the CSR/CSC graph container.
+//
+// Mirrors graph_base / graph_t<CSR> / graph_t<CSC> (reference
+// include/graph.cuh:113-321) and graph_loader (include/graph_loader.cuh).
+// Differences by design: offsets are u64; the graph owns and frees its
+// memory; placement is an explicit mode (HBM / managed / capped) instead of
+// Set_Mem_Policy's flags over managed memory (include/graph.cuh:170-251);
+// for symmetric graphs the CSC aliases the CSR instead of a host transpose.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace mgg {
+
+enum class Mem { None, Device, Managed, HostPinned };
+
+// Owning untyped buffer.
+struct Buf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  Mem kind = Mem::None;
+  Buf() = default;
+  Buf(const Buf &) = delete;
+  Buf &operator=(const Buf &) = delete;
+  Buf(Buf &&o) noexcept { *this = std::move(o); }
+  Buf &operator=(Buf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      kind = o.kind;
+      o.p = nullptr;
+      o.bytes = 0;
+      o.kind = Mem::None;
+    }
+    return *this;
+  }
+  ~Buf() { release(); }
+  void alloc(size_t nbytes, Mem k = Mem::Device);
+  // grow-only (contents not preserved)
+  void ensure(size_t nbytes, Mem k = Mem::Device) {
+    if (nbytes > bytes || kind != k) alloc(nbytes, k);
+  }
+  void release();
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cu_count = 256;
+  grus_tuning tuning{};
+  uint64_t *h_ctl = nullptr;  // pinned host mirror of the control block
+  Buf d_ctl;                  // device control block (counters)
+  Buf scan_tmp;               // rocPRIM temporary storage, grown on demand
+  explicit Context(int dev);
+  ~Context();
+  void sync() { GRUS_HIP(hipStreamSynchronize(stream)); }
+  void activate() const { GRUS_HIP(hipSetDevice(device)); }
+  // copy the first n u64 of the device control block to h_ctl and wait
+  void read_ctl(size_t n);
+};
+
+// POD view of a CSR (or CSC) handed to kernels.  Edges [0, split) are read
+// from col/w; edges [split, E) from col_far/w_far (host-resident remainder
+// in capped placement; split == E otherwise).
+struct CsrView {
+  const uint64_t *off;
+  const uint32_t *col;
+  const uint32_t *w;
+  const uint32_t *col_far;
+  const uint32_t *w_far;
+  uint64_t split;
+  uint64_t V, E;
+};
+
+struct BfsWork;
+struct SsspWork;
+struct PrWork;
+struct CcWork;
+struct MsWork;
+
+struct Graph {
+  Context *ctx = nullptr;
+  uint64_t V = 0, E = 0;
+  bool weighted = false;
+  bool symmetric = false;
+  int placement = GRUS_PLACE_HBM;
+  Buf off, col, w;           // CSR (device, managed, or HBM prefix in capped mode)
+  Buf col_far, w_far;        // capped mode: host-resident edge remainder
+  uint64_t split = 0;        // first edge held in col_far
+  Buf in_off, in_col, in_w;  // CSC of a directed graph (built on first pull use)
+  bool csc_ready = false;
+  // derived at creation
+  uint32_t max_deg_vertex = 0;
+  uint64_t max_deg = 0;
+  uint64_t isolated = 0;
+  Buf unreachable_bm;  // bit v set: in-degree 0 (never discovered by a traversal)
+  bool unreachable_ready = false;
+  uint64_t host_bytes = 0;
+  // workspaces and last result
+  Buf result;
+  std::shared_ptr<BfsWork> bfs;
+  std::shared_ptr<SsspWork> sssp;
+  std::shared_ptr<PrWork> pr;
+  std::shared_ptr<CcWork> cc;
+  std::shared_ptr<MsWork> ms;
+
+  explicit Graph(Context *c);
+  ~Graph();
+  CsrView csr() const;
+  CsrView csc();  // in-edges; aliases csr() when symmetric
+  uint64_t words() const { return (V + 63) / 64; }
+  void compute_derived();         // max degree vertex, isolated count
+  const uint64_t *unreachable();  // in-degree-0 bitmap (builds CSC if directed)
+  uint64_t device_bytes() const;
+};
+
+// graph construction (graph.hip)
+std::unique_ptr<Graph> graph_from_host(Context *ctx, uint64_t V, uint64_t E, const uint64_t *off,
+                                       const uint32_t *col, const uint32_t *w, bool symmetric,
+                                       const grus_place *place);
+std::unique_ptr<Graph> graph_gen_rmat(Context *ctx, int scale, int ef, double a, double b,
+                                      double c, uint64_t seed, uint32_t wlo, uint32_t whi,
+                                      uint64_t wseed, const grus_place *place);
+std::unique_ptr<Graph> graph_gen_grid(Context *ctx, uint32_t rows, uint32_t cols, uint32_t wlo,
+                                      uint32_t whi, uint64_t wseed, const grus_place *place);
+void graph_copy_csr(Graph &g, uint64_t *off, uint32_t *col, uint32_t *w);
+// (re)place a device-built CSR per the placement request
+void graph_apply_placement(Graph &g, const grus_place *place);
+
+// .gr I/O (gr_io.cpp; host only)
+struct GrHeader {
+  uint64_t V, E, size_edge_ty;
+};
+GrHeader gr_read_header(const std::string &path);
+void gr_read(const std::string &path, uint64_t *off, uint32_t *col, uint32_t *w);
+void gr_write(const std::string &path, uint64_t V, uint64_t E, const uint64_t *off,
+              const uint32_t *col, const uint32_t *w);
+
+}  // namespace mgg

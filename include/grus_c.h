@@ -1,0 +1,221 @@
+/*
+ * grus_c.h -- C-ABI of libgrus_hip.so, the MI355X (gfx950) traversal engine.
+ *
+ * The reference (wpybtw/Grus_framework) exposes its engine only as CUDA C++
+ * header templates; there is no FFI.  Each entry point below replaces the
+ * reference interface named in its comment (paths relative to the reference
+ * root).  Inside the library the reference's operator surface is kept:
+ * graph / frontier / job classes and app functors (updater, generator,
+ * pull_selector) compiled into the HIP translation units.
+ *
+ * Conventions
+ *   - Every function returns GRUS_OK (0) or a negative GRUS_E_* code and
+ *     never exits the process (the reference's H_ERR calls exit(),
+ *     include/common.cuh:57-68).  grus_last_error() gives a thread-local
+ *     message for the last failure on the calling thread.
+ *   - Handles own all device memory they allocate; the caller owns every
+ *     host buffer it passes.  Inputs are copied in at create time; results
+ *     are copied out before a call returns (output pointers may be NULL to
+ *     leave the result on the device, see grus_result_copy).
+ *   - One context = one device + one HIP stream.  Calls on one context or on
+ *     graphs of one context are serialized by the caller (not reentrant).
+ *     A server (grus_server_*) owns its own worker thread and streams.
+ *   - Vertex ids are uint32; edge offsets are uint64 (the reference's u32
+ *     xadj, include/graph_loader.cuh:118, overflows past 2^32 edges).
+ *   - Unreached vertices hold GRUS_INFINIT (0xFFFFFFFE, the reference's
+ *     INFINIT, include/common.cuh:52).
+ */
+#ifndef GRUS_C_H
+#define GRUS_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRUS_INFINIT 0xFFFFFFFEu
+
+/* status codes */
+#define GRUS_OK 0
+#define GRUS_E_INVALID -1     /* bad argument */
+#define GRUS_E_NOMEM -2       /* device or host allocation failed */
+#define GRUS_E_HIP -3         /* HIP runtime error */
+#define GRUS_E_IO -4          /* file open/read/write failure */
+#define GRUS_E_FORMAT -5      /* malformed .gr file */
+#define GRUS_E_BUDGET -6      /* request exceeds the memory budget / cap */
+#define GRUS_E_TIMEOUT -7     /* deadline passed */
+#define GRUS_E_UNSUPPORTED -8 /* operation not available for this graph */
+#define GRUS_E_NODEVICE -9    /* no usable GPU */
+#define GRUS_E_CLOSED -10     /* server shut down */
+
+typedef struct grus_ctx grus_ctx;
+typedef struct grus_graph grus_graph;
+typedef struct grus_server grus_server;
+
+/* Placement of graph arrays.  Replaces graph_base::Set_Mem_Policy
+ * (include/graph.cuh:170-251) and its --opt/--pf/--ab/--rm/--pl flags
+ * (include/app.cuh:27-31). */
+#define GRUS_PLACE_HBM 0      /* hipMalloc: whole CSR resident in HBM */
+#define GRUS_PLACE_MANAGED 1  /* hipMallocManaged + advice/prefetch per flags */
+#define GRUS_PLACE_CAPPED 2   /* offsets + edge prefix in HBM up to hbm_cap,
+                                 remaining edges in host-coherent pinned memory */
+typedef struct grus_place {
+  int mode;               /* GRUS_PLACE_* */
+  uint64_t hbm_cap_bytes; /* CAPPED: device bytes the graph may use (0 = no cap) */
+  int prefetch;           /* MANAGED: --pf  prefetch arrays to the device */
+  int accessed_by;        /* MANAGED: --ab  hipMemAdviseSetAccessedBy */
+  int read_mostly;        /* MANAGED: --rm  hipMemAdviseSetReadMostly */
+  int preferred_location; /* MANAGED: --pl  hipMemAdviseSetPreferredLocation */
+} grus_place;
+
+/* Direction policy for BFS (samples/bfs/bfs.cu:135-173 push, :104-134 pull) */
+#define GRUS_DIR_AUTO 0 /* direction-optimizing: push <-> pull switch */
+#define GRUS_DIR_PUSH 1 /* top-down only (the reference's "plain method") */
+#define GRUS_DIR_PULL 2 /* bottom-up only */
+
+typedef struct grus_stats {
+  double device_ms;         /* device time of the traversal (HIP events) */
+  uint32_t iterations;      /* levels / rounds / PR iterations */
+  uint32_t pull_iterations; /* how many of them ran bottom-up (pull) */
+  uint64_t vertices_reached; /* V_r */
+  uint64_t edges_reached;    /* E_r: sum of out-degrees of reached vertices */
+  uint64_t kernel_launches;
+  uint64_t reserved[4];
+} grus_stats;
+
+/* Tunables of the traversal engine (the autotuner writes these). */
+typedef struct grus_tuning {
+  double bfs_alpha;       /* push->pull when m_f > m_u / alpha   (default 14) */
+  double bfs_beta;        /* pull->push when n_f < V / beta      (default 24) */
+  uint32_t bu_thread_edges; /* bottom-up: edges a lane scans before handing
+                               the vertex to a wave (default 32) */
+  uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic) */
+  uint32_t reserved[8];
+} grus_tuning;
+
+/* ---- errors / version ---- */
+const char *grus_last_error(void);
+const char *grus_version(void);
+
+/* ---- context ---- */
+/* Replaces cudaSetDevice(FLAGS_device) + implicit default stream
+ * (samples/bfs/bfs.cu:139, include/app.cuh:16). */
+int grus_init(int device, grus_ctx **out);
+int grus_fini(grus_ctx *ctx);
+int grus_device_count(int *count);
+int grus_get_tuning(grus_ctx *ctx, grus_tuning *out);
+int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in);
+
+/* ---- host-side .gr I/O (no device needed) ----
+ * Galois .gr v1: u64 header {1, sizeEdgeTy, N, M}, u64[N] end offsets,
+ * u32[M] destinations, 4-byte pad if M odd and weighted, u32[M] weights.
+ * Replaces graph_loader::ReadGraphGR (include/graph_loader.cuh:65-159).
+ * grus_gr_read fills caller buffers: off[V+1] (off[0]=0), col[E], and
+ * w[E] when w != NULL (all ones if the file has no weights). */
+int grus_gr_header(const char *path, uint64_t *V, uint64_t *E, int *weighted);
+int grus_gr_read(const char *path, uint64_t *off, uint32_t *col, uint32_t *w);
+int grus_gr_write(const char *path, uint64_t V, uint64_t E, const uint64_t *off,
+                  const uint32_t *col, const uint32_t *w /* NULL = unweighted */);
+
+/* ---- graphs ----
+ * symmetric != 0 declares that the CSR equals its transpose (then the CSC
+ * used by pull kernels aliases the CSR); otherwise the CSC is built on the
+ * device when a pull traversal first needs it, weights carried (the
+ * reference's csr_tocsc drops them, include/graph.cuh:294). */
+/* graph_t<CSR> from caller arrays (include/graph.cuh:254-268) */
+int grus_graph_from_csr(grus_ctx *ctx, uint64_t V, uint64_t E, const uint64_t *off,
+                        const uint32_t *col, const uint32_t *w, int symmetric,
+                        const grus_place *place, grus_graph **out);
+/* graph_loader::Load (include/graph_loader.cuh:160-166) + Set_Mem_Policy */
+int grus_graph_load_gr(grus_ctx *ctx, const char *path, int need_weight, int symmetric,
+                       const grus_place *place, grus_graph **out);
+/* Synthetic inputs generated on the device (the reference has none):
+ * RMAT(scale, edge_factor, a, b, c, seed) symmetrized, deduplicated,
+ * self-loops dropped, rows sorted; weights (if wlo>0) uniform in [wlo,whi]
+ * per undirected edge from wseed.  Grid: rows x cols 4-neighbour lattice,
+ * id = r*cols + c. */
+int grus_graph_gen_rmat(grus_ctx *ctx, int scale, int edge_factor, double a, double b,
+                        double c, uint64_t seed, uint32_t wlo, uint32_t whi,
+                        uint64_t wseed, const grus_place *place, grus_graph **out);
+int grus_graph_gen_grid(grus_ctx *ctx, uint32_t rows, uint32_t cols, uint32_t wlo,
+                        uint32_t whi, uint64_t wseed, const grus_place *place,
+                        grus_graph **out);
+typedef struct grus_graph_info {
+  uint64_t V, E;
+  int weighted, symmetric, placement;
+  uint32_t max_degree_vertex; /* lowest id among max out-degree (loader :132-137) */
+  uint64_t max_degree;
+  uint64_t isolated;          /* vertices with out-degree 0 */
+  uint64_t device_bytes;      /* HBM held by the graph */
+  uint64_t host_bytes;        /* host-resident edge bytes (CAPPED/MANAGED) */
+} grus_graph_info;
+int grus_graph_info_get(grus_graph *g, grus_graph_info *out);
+/* Copy the CSR back to host buffers (any pointer may be NULL). */
+int grus_graph_copy_csr(grus_graph *g, uint64_t *off, uint32_t *col, uint32_t *w);
+int grus_graph_destroy(grus_graph *g);
+
+/* ---- apps (single source) ---- */
+/* BFS levels.  Replaces BFS_single_gpu / BFS_pull_single_gpu
+ * (samples/bfs/bfs.cu:104-173) with kernel<...Frontier<BDF>...> /
+ * kernel_pull (include/kernel.cuh:124-165). */
+int grus_bfs(grus_graph *g, uint32_t src, int dir_mode, uint32_t *levels_out,
+             grus_stats *stats);
+/* SSSP distances (samples/sssp/sssp.cu:123-170; the relax is an atomic
+ * min here, fixing the lost update at :54-55).  Requires weights (or unit
+ * weights when the graph has none). */
+int grus_sssp(grus_graph *g, uint32_t src, uint32_t *dist_out, grus_stats *stats);
+/* PageRank pull, FP32, fixed iterations (samples/pagerank/pr.cu:134-168,
+ * redefined: r0=(1-a)/V, r'=(1-a)/V + a*sum r(u)/outdeg(u)). */
+int grus_pagerank(grus_graph *g, float alpha, int iters, float *rank_out,
+                  grus_stats *stats);
+/* Connected components: label = min vertex id of the component
+ * (samples/cc/cc.cu:155-190).  method 0 = hooking + pointer jumping,
+ * 1 = min-label propagation through the push operator (the reference's
+ * algorithm). */
+int grus_cc(grus_graph *g, int method, uint32_t *labels_out, grus_stats *stats);
+/* Copy the last app result held on the device for g (V elements of 4 B). */
+int grus_result_copy(grus_graph *g, void *out);
+
+/* ---- batched multi-source (no reference counterpart: the north-star
+ * batching pattern; each source gives exactly the single-source result) ----
+ * k <= 64 sources packed as bits of one u64 per vertex.  out is k*V u32
+ * (row i = result of srcs[i]); NULL keeps results on the device. */
+int grus_bfs_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
+                   grus_stats *stats);
+int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
+                    grus_stats *stats);
+
+/* ---- serving (no reference counterpart) ----
+ * A server keeps the graph and per-query state on the device, admits
+ * queries as lane slots and state memory free up, batches up to 64 queries
+ * of one kind per traversal, and keeps admitting while batches run. */
+#define GRUS_Q_BFS 0
+#define GRUS_Q_SSSP 1
+typedef struct grus_server_opts {
+  int max_inflight_batches;   /* concurrent batches (streams), default 2 */
+  int max_batch;              /* lanes per batch, 1..64, default 64 */
+  uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
+  int copy_results;           /* 1: copy each result to the host buffer given
+                                 at submit; 0: keep a digest only */
+} grus_server_opts;
+typedef struct grus_query_result {
+  uint64_t id;
+  int status;            /* GRUS_OK or error */
+  uint64_t reached;      /* vertices with a finite value */
+  uint64_t checksum;     /* FNV-1a 64 of the V-element result */
+  double submit_us, start_us, done_us; /* server clock */
+} grus_query_result;
+int grus_server_create(grus_graph *g, const grus_server_opts *opts, grus_server **out);
+/* out_host: V u32 for the result when copy_results=1 (may be NULL). */
+int grus_server_submit(grus_server *s, int kind, uint32_t src, uint32_t *out_host,
+                       uint64_t *id_out);
+/* Wait up to timeout_ms (<0 = forever) for one finished query. */
+int grus_server_poll(grus_server *s, grus_query_result *res, int timeout_ms);
+int grus_server_destroy(grus_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRUS_C_H */
