@@ -76,7 +76,39 @@ __global__ void k_fill_u64(uint64_t *a, uint64_t n, uint64_t v) {
   for (; i < n; i += stride) a[i] = v;
 }
 
+__global__ void k_reached(const uint64_t *off, const uint32_t *val, uint64_t V,
+                          unsigned long long *acc) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long n = 0, e = 0;
+  for (; i < V; i += stride) {
+    if (val[i] != INFINIT) {
+      n++;
+      e += off[i + 1] - off[i];
+    }
+  }
+  n = wave_sum(n);
+  e = wave_sum(e);
+  if (lane_id() == 0) {
+    atomicAdd(&acc[0], n);
+    atomicAdd(&acc[1], e);
+  }
+}
+
 }  // namespace
+
+void reached_stats(Context &ctx, const uint64_t *off, const uint32_t *val, uint64_t V,
+                   uint64_t *vr, uint64_t *er) {
+  auto *acc = reinterpret_cast<unsigned long long *>(ctx.d_ctl.p) + 40;
+  GRUS_HIP(hipMemsetAsync(acc, 0, 16, ctx.stream));
+  if (V) {
+    k_reached<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(off, val, V, acc);
+    GRUS_LAUNCH_CHECK();
+  }
+  ctx.read_ctl(42);
+  *vr = ctx.h_ctl[40];
+  *er = ctx.h_ctl[41];
+}
 
 unsigned grid_cap(Context &ctx, uint64_t items, unsigned per_block, unsigned blocks_per_cu) {
   uint64_t g = div_up(items ? items : 1, per_block);

@@ -13,8 +13,9 @@
 //
 // Functor contract kept from the reference (samples/bfs/bfs.cu:51-87):
 //   bool Job::update(src, dst, edge_id, weight)  -- the app's updater
-//   bool Job::generate(updated, dst)             -- the app's generator:
-//        true when dst joins the next frontier (deduplicated)
+//   int  Job::generate(updated, dst)             -- the app's generator:
+//        1 when dst joins the next frontier (deduplicated), 2 when it joins
+//        the far queue (Job::kFar apps only), 0 otherwise
 // The operator appends generated vertices to the next queue with one
 // atomic per wave and counts the next frontier's size and out-edges (the
 // inputs of the push/pull direction heuristic).
@@ -30,7 +31,8 @@ struct StepCounters {
   uint32_t next_count;   // vertices appended to the next queue
   uint32_t long_count;   // bottom-up vertices handed to wave-cooperative scan
   uint64_t next_edges;   // sum of out-degrees of the next frontier
-  uint64_t edges_seen;   // edges examined (diagnostics)
+  uint32_t far_count;    // vertices appended to the far queue (SSSP)
+  uint32_t min_far;      // minimum live far distance (SSSP threshold advance)
   uint64_t pad[5];
 };
 
@@ -52,6 +54,9 @@ void queue_to_bitmap(Context &ctx, const uint32_t *q, uint32_t n, uint64_t *bm, 
 void fill_u32(Context &ctx, uint32_t *a, uint64_t n, uint32_t v);
 void fill_u64(Context &ctx, uint64_t *a, uint64_t n, uint64_t v);
 unsigned grid_cap(Context &ctx, uint64_t items, unsigned per_block, unsigned blocks_per_cu = 16);
+// V_r = #{v : val[v] != INFINIT}, E_r = sum of out-degrees over them (synchronous)
+void reached_stats(Context &ctx, const uint64_t *off, const uint32_t *val, uint64_t V,
+                   uint64_t *vr, uint64_t *er);
 
 constexpr int LB_CHUNK = 1024;          // edges per workgroup step (4 per lane)
 constexpr int LB_WIN = LB_CHUNK + 64;   // queue entries staged in LDS
@@ -72,7 +77,8 @@ __device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t 
 template <class Job>
 __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
                                                    const uint64_t *S, uint32_t F, uint64_t total,
-                                                   Job job, uint32_t *qn, StepCounters *cnt) {
+                                                   Job job, uint32_t *qn, uint32_t *qfar,
+                                                   StepCounters *cnt) {
   __shared__ uint64_t sS[LB_WIN];
   __shared__ uint64_t sRow[LB_WIN];
   __shared__ uint32_t sQ[LB_WIN];
@@ -104,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
     for (uint64_t kb = c0; kb < c1; kb += blockDim.x) {
       uint64_t k = kb + threadIdx.x;
       bool valid = k < c1;
-      bool gen = false;
+      int gen = 0;
       uint32_t dst = 0;
       if (valid) {
         uint32_t j, u;
@@ -125,10 +131,14 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
         bool upd = job.update(u, dst, e, edge_w(g, e));
         gen = job.generate(upd, dst);
       }
-      uint32_t slot = wave_append(&cnt->next_count, gen);
-      if (gen) {
+      uint32_t slot = wave_append(&cnt->next_count, gen == 1);
+      if (gen == 1) {
         qn[slot] = dst;
         my_edges += job.out_degree(dst);
+      }
+      if (Job::kFar) {
+        uint32_t fslot = wave_append(&cnt->far_count, gen == 2);
+        if (gen == 2) qfar[fslot] = dst;
       }
     }
     __syncthreads();
@@ -153,10 +163,11 @@ unsigned push_grid(Context &ctx, uint64_t total);
 
 template <class Job>
 void push_advance(Context &ctx, PushWork &w, const CsrView &g, const uint32_t *q, uint32_t F,
-                  uint64_t total, const Job &job, uint32_t *qn, StepCounters *cnt) {
+                  uint64_t total, const Job &job, uint32_t *qn, StepCounters *cnt,
+                  uint32_t *qfar = nullptr) {
   if (F == 0 || total == 0) return;
   k_push_lb<Job><<<push_grid(ctx, total), BLOCK, 0, ctx.stream>>>(
-      g, q, w.scan.as<uint64_t>(), F, total, job, qn, cnt);
+      g, q, w.scan.as<uint64_t>(), F, total, job, qn, qfar, cnt);
   GRUS_LAUNCH_CHECK();
 }
 

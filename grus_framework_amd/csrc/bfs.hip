@@ -56,6 +56,7 @@ namespace bfs {
 
 // job_t of samples/bfs/bfs.cu:25-49 as the device-side job of one level
 struct job_t {
+  static constexpr bool kFar = false;
   uint32_t *label;
   uint64_t *visited;
   const uint64_t *out_off;
@@ -69,7 +70,7 @@ struct job_t {
     return true;
   }
   // generator (bfs.cu:67-71): a claimed vertex joins the next frontier once
-  __device__ __forceinline__ bool generate(bool updated, vtx_t) const { return updated; }
+  __device__ __forceinline__ int generate(bool updated, vtx_t) const { return updated ? 1 : 0; }
   __device__ __forceinline__ uint64_t out_degree(vtx_t v) const {
     return out_off[v + 1] - out_off[v];
   }

@@ -3,11 +3,6 @@
 #include "serve.hpp"
 
 namespace mgg {
-void run_sssp(Graph &, uint32_t, grus_stats *) { raise(GRUS_E_UNSUPPORTED, "sssp not built"); }
-void run_pagerank(Graph &, float, int, grus_stats *) {
-  raise(GRUS_E_UNSUPPORTED, "pagerank not built");
-}
-void run_cc(Graph &, int, grus_stats *) { raise(GRUS_E_UNSUPPORTED, "cc not built"); }
 void run_bfs_multi(Graph &, const uint32_t *, int, uint32_t *, grus_stats *) {
   raise(GRUS_E_UNSUPPORTED, "multi-source bfs not built");
 }

@@ -1,0 +1,279 @@
+// sssp.hip -- single-source shortest paths, near-far delta stepping.
+//
+// Reference: samples/sssp/sssp.cu (SSSPInit :18-23, job_t :24-49, updater
+// :51-60, push driver :123-170).  The reference relaxes with a non-atomic
+// check-then-store (sssp.cu:54-55), which can lose an update; here the
+// relax is atomicMin, so the fixed point -- the unique shortest distances --
+// is reached whatever the interleaving and the result is bit-exact.
+//
+// Schedule (Davidson et al., IPDPS'14 "near-far"): vertices whose tentative
+// distance is below the threshold T form the near frontier and are relaxed
+// by the edge-balanced push operator round after round; improved vertices
+// at or beyond T go to a far queue.  When the near frontier empties, T
+// advances to the bucket holding the smallest live far distance and the far
+// queue is split.  Queue membership is deduplicated with per-vertex round
+// stamps (u32), which never need clearing between rounds.
+#include <algorithm>
+
+#include "advance.hpp"
+#include "apps.hpp"
+
+namespace mgg {
+
+struct SsspWork {
+  Buf q[2], far[2], near_stamp, far_stamp;
+  PushWork push;
+  uint64_t cap_v = 0;
+  uint32_t stamp = 0;  // last stamp handed out; stamps only grow
+  double mean_w = -1;
+  void ensure(Context &ctx, uint64_t V) {
+    if (V <= cap_v) return;
+    q[0].alloc((V + 1) * 4);
+    q[1].alloc((V + 1) * 4);
+    far[0].alloc((V + 1) * 4);
+    far[1].alloc((V + 1) * 4);
+    near_stamp.alloc((V + 1) * 4);
+    far_stamp.alloc((V + 1) * 4);
+    fill_u32(ctx, near_stamp.as<uint32_t>(), V + 1, 0);
+    fill_u32(ctx, far_stamp.as<uint32_t>(), V + 1, 0);
+    push.ensure(V);
+    cap_v = V;
+    stamp = 0;
+  }
+  uint32_t next_stamp(Context &ctx, uint64_t V) {
+    if (stamp >= 0xFFFFFF00u) {  // wrap: clear and restart
+      fill_u32(ctx, near_stamp.as<uint32_t>(), V + 1, 0);
+      fill_u32(ctx, far_stamp.as<uint32_t>(), V + 1, 0);
+      stamp = 0;
+    }
+    return ++stamp;
+  }
+};
+
+namespace sssp {
+
+// job_t of samples/sssp/sssp.cu:24-49 as the device-side job of one round
+struct job_t {
+  static constexpr bool kFar = true;
+  uint32_t *dist;
+  uint32_t *near_stamp;
+  uint32_t *far_stamp;
+  const uint64_t *out_off;
+  uint32_t T;           // near threshold (exclusive)
+  uint32_t near_id;     // stamp of the near queue being built
+  uint32_t far_id;      // stamp of the current far epoch
+  // updater (sssp.cu:51-60) with an atomic min
+  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t w) const {
+    uint64_t nd = (uint64_t)__atomic_load_n(&dist[src], __ATOMIC_RELAXED) + w;
+    if (nd >= INFINIT) return false;  // distances are u32 below INFINIT
+    return atomicMin(&dist[dst], (uint32_t)nd) > (uint32_t)nd;
+  }
+  // generator: an improved vertex joins the near queue (below T) or the far
+  // queue, once per queue
+  __device__ __forceinline__ int generate(bool updated, vtx_t dst) const {
+    if (!updated) return 0;
+    uint32_t d = __atomic_load_n(&dist[dst], __ATOMIC_RELAXED);
+    if (d < T) return atomicExch(&near_stamp[dst], near_id) != near_id ? 1 : 0;
+    return atomicExch(&far_stamp[dst], far_id) != far_id ? 2 : 0;
+  }
+  __device__ __forceinline__ uint64_t out_degree(vtx_t v) const {
+    return out_off[v + 1] - out_off[v];
+  }
+};
+
+__global__ void k_init(uint32_t *dist, uint64_t V, uint32_t src, uint32_t *q0) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V; i += stride) {
+    dist[i] = (i == src) ? 0u : INFINIT;  // SSSPInit sssp.cu:21
+    if (i == 0) q0[0] = src;
+  }
+}
+
+// smallest distance >= T among far entries
+__global__ void k_far_min(const uint32_t *far, uint32_t n, const uint32_t *dist, uint32_t T,
+                          uint32_t *out) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t m = 0xFFFFFFFFu;
+  for (; i < n; i += stride) {
+    uint32_t d = dist[far[i]];
+    if (d >= T && d < m) m = d;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t x = __shfl_xor(m, o);
+    m = x < m ? x : m;
+  }
+  if (lane_id() == 0 && m != 0xFFFFFFFFu) atomicMin(out, m);
+}
+
+// split the far queue at the new threshold: [T_old, T) -> near, >= T -> far'
+// (entries below T_old were already relaxed at their final near distance)
+__global__ void k_far_split(const uint32_t *far, uint32_t n, const uint32_t *dist,
+                            const uint64_t *out_off, uint32_t T_old, uint32_t T,
+                            uint32_t *near_stamp, uint32_t near_id, uint32_t *far_stamp,
+                            uint32_t far_id, uint32_t *qn, uint32_t *far_out,
+                            StepCounters *cnt) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t first = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~(uint64_t)(WAVE - 1));
+  uint64_t my_edges = 0;
+  for (uint64_t base = first; base < n; base += stride) {
+    uint64_t i = base + lane_id();
+    int to = 0;
+    uint32_t v = 0;
+    if (i < n) {
+      v = far[i];
+      uint32_t d = dist[v];
+      if (d >= T_old) {
+        if (d < T) to = atomicExch(&near_stamp[v], near_id) != near_id ? 1 : 0;
+        else to = atomicExch(&far_stamp[v], far_id) != far_id ? 2 : 0;
+      }
+    }
+    uint32_t s1 = wave_append(&cnt->next_count, to == 1);
+    if (to == 1) {
+      qn[s1] = v;
+      my_edges += out_off[v + 1] - out_off[v];
+    }
+    uint32_t s2 = wave_append(&cnt->far_count, to == 2);
+    if (to == 2) far_out[s2] = v;
+  }
+  my_edges = wave_sum(my_edges);
+  if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+}
+
+__global__ void k_weight_sum(const uint32_t *w, uint64_t E, unsigned long long *out) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long s = 0;
+  for (; i < E; i += stride) s += w[i];
+  s = wave_sum(s);
+  if (lane_id() == 0) atomicAdd(out, s);
+}
+
+}  // namespace sssp
+
+namespace {
+uint32_t pick_delta(Graph &g, SsspWork &w) {
+  Context &ctx = *g.ctx;
+  if (ctx.tuning.sssp_delta) return ctx.tuning.sssp_delta;
+  if (!g.weighted) return 0xFFFFFFFFu;  // unit weights: one bucket (BFS order)
+  if (w.mean_w < 0) {
+    double mean = 0;
+    // the HBM-resident edges are a fair sample in capped placement
+    uint64_t n = g.placement == GRUS_PLACE_CAPPED ? g.split : g.E;
+    if (n) {
+      auto *acc = reinterpret_cast<unsigned long long *>(ctx.d_ctl.p) + 32;
+      GRUS_HIP(hipMemsetAsync(acc, 0, 8, ctx.stream));
+      sssp::k_weight_sum<<<grid_cap(ctx, n, BLOCK), BLOCK, 0, ctx.stream>>>(
+          g.w.as<uint32_t>(), n, acc);
+      GRUS_LAUNCH_CHECK();
+      ctx.read_ctl(33);
+      mean = (double)ctx.h_ctl[32] / (double)n;
+    }
+    w.mean_w = mean > 0 ? mean : 1.0;
+  }
+  double avg_deg = g.V ? (double)g.E / (double)g.V : 1.0;
+  double d = w.mean_w * std::max(2.0, 32.0 / std::max(avg_deg, 1.0));
+  return (uint32_t)std::min(std::max(d, 1.0), 4.0e9);
+}
+}  // namespace
+
+void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
+  Context &ctx = *g.ctx;
+  ctx.activate();
+  if (src >= g.V) raise(GRUS_E_INVALID, "source vertex out of range");
+  if (!g.sssp) g.sssp = std::make_shared<SsspWork>();
+  SsspWork &w = *g.sssp;
+  w.ensure(ctx, g.V);
+  g.result.ensure(g.V * 4);
+  const uint64_t V = g.V;
+  const CsrView out = g.csr();
+  uint32_t *dist = g.result.as<uint32_t>();
+  auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
+  const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
+  const uint64_t delta = pick_delta(g, w);
+
+  uint64_t src_deg;
+  {
+    uint64_t o[2];
+    GRUS_HIP(hipMemcpy(o, g.off.as<uint64_t>() + src, 16, hipMemcpyDeviceToHost));
+    src_deg = o[1] - o[0];
+  }
+  uint64_t launches = 0, rounds = 0, relaxed_vertices = 0;
+  GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  sssp::k_init<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(dist, V, src,
+                                                                 w.q[0].as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  launches++;
+  int cur = 0, fcur = 0;
+  uint64_t nn = 1, mf = src_deg, nfar = 0;
+  uint64_t T = delta;
+  uint32_t far_id = w.next_stamp(ctx, V);
+  const uint64_t round_cap = 64 * V + 1024;
+  for (;;) {
+    while (nn) {
+      if (++rounds > round_cap) raise(GRUS_E_HIP, "sssp did not converge");
+      relaxed_vertices += nn;
+      GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+      push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nn);
+      sssp::job_t job{dist,
+                      w.near_stamp.as<uint32_t>(),
+                      w.far_stamp.as<uint32_t>(),
+                      out.off,
+                      (uint32_t)std::min<uint64_t>(T, 0xFFFFFFFFull),
+                      w.next_stamp(ctx, V),
+                      far_id};
+      push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nn, mf, job,
+                   w.q[cur ^ 1].as<uint32_t>(), cnt, w.far[fcur].as<uint32_t>() + nfar);
+      launches += 4;
+      ctx.read_ctl(3);
+      nn = hc->next_count;
+      mf = hc->next_edges;
+      nfar += hc->far_count;
+      cur ^= 1;
+    }
+    if (nfar == 0) break;
+    // advance the threshold to the bucket of the smallest live far distance
+    GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+    GRUS_HIP(hipMemsetAsync(&cnt->min_far, 0xFF, 4, ctx.stream));
+    uint32_t Tk = (uint32_t)std::min<uint64_t>(T, 0xFFFFFFFFull);
+    sssp::k_far_min<<<grid_cap(ctx, nfar, BLOCK), BLOCK, 0, ctx.stream>>>(
+        w.far[fcur].as<uint32_t>(), (uint32_t)nfar, dist, Tk, &cnt->min_far);
+    GRUS_LAUNCH_CHECK();
+    ctx.read_ctl(3);
+    uint32_t mn = hc->min_far;
+    if (mn == 0xFFFFFFFFu) break;  // every far entry was stale
+    uint64_t T_new = ((uint64_t)mn / delta + 1) * delta;
+    far_id = w.next_stamp(ctx, V);
+    uint32_t near_id = w.next_stamp(ctx, V);
+    GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+    sssp::k_far_split<<<grid_cap(ctx, nfar, BLOCK), BLOCK, 0, ctx.stream>>>(
+        w.far[fcur].as<uint32_t>(), (uint32_t)nfar, dist, out.off, Tk,
+        (uint32_t)std::min<uint64_t>(T_new, 0xFFFFFFFFull), w.near_stamp.as<uint32_t>(),
+        near_id, w.far_stamp.as<uint32_t>(), far_id, w.q[cur].as<uint32_t>(),
+        w.far[fcur ^ 1].as<uint32_t>(), cnt);
+    GRUS_LAUNCH_CHECK();
+    launches += 3;
+    ctx.read_ctl(3);
+    nn = hc->next_count;
+    mf = hc->next_edges;
+    nfar = hc->far_count;
+    fcur ^= 1;
+    T = T_new;
+  }
+  GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  GRUS_HIP(hipEventSynchronize(ctx.ev1));
+  float ms = 0;
+  GRUS_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+  if (st) {
+    *st = grus_stats{};
+    reached_stats(ctx, out.off, dist, V, &st->vertices_reached, &st->edges_reached);
+    st->device_ms = ms;
+    st->iterations = (uint32_t)rounds;
+    st->kernel_launches = launches;
+    st->reserved[0] = relaxed_vertices;
+    st->reserved[1] = delta;
+  }
+}
+
+}  // namespace mgg

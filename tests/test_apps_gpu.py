@@ -1,0 +1,208 @@
+"""GPU parity: SSSP distances and CC labels bit-exact, PageRank L1 <= 1e-6 vs FP64."""
+import numpy as np
+import pytest
+
+import grus_framework_amd as grus
+from grus_framework_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+INF = grus.INFINIT
+PR_TOL = 1e-6  # L1 over normalised ranks vs the FP64 oracle
+
+
+def host_csr(g):
+    from oracle import oracle_py as O
+    off, col, w = g.csr()
+    return O.CSR(off, col, w)
+
+
+def random_directed(rng, V, M, wmax=None):
+    u = rng.integers(0, V, M)
+    v = rng.integers(0, V, M)
+    key = np.unique(u.astype(np.uint64) << np.uint64(32) | v.astype(np.uint64))
+    key = key[(key >> np.uint64(32)) != (key & np.uint64(0xFFFFFFFF))]
+    src = (key >> np.uint64(32)).astype(np.int64)
+    col = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    off = np.zeros(V + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    w = None if wmax is None else rng.integers(1, wmax + 1, len(col)).astype(np.uint32)
+    return off, col, w
+
+
+# ------------------------------------------------------------------ SSSP
+@pytest.mark.parametrize("scale", [10, 15])
+def test_sssp_rmat_vs_oracle(ctx, oracle, scale):
+    with grus.Graph.rmat(ctx, scale, weights=(1, 255), wseed=3) as g:
+        h = host_csr(g)
+        deg = h.degrees()
+        srcs = [int(np.argmax(deg)), int(np.nonzero(deg)[0][5]), int(np.nonzero(deg == 0)[0][0])]
+        for s in srcs:
+            print("sssp rmat", scale, "src", s, flush=True)
+            d, st = grus.sssp(g, s)
+            ref = oracle.sssp(h, s)
+            assert (d == ref).all(), s
+            reached = ref != INF
+            assert st.vertices_reached == reached.sum()
+            assert st.edges_reached == deg[reached].sum()
+
+
+def test_sssp_grid_vs_oracle_and_delta_invariance(ctx, oracle):
+    with grus.Graph.grid(ctx, 256, 320, weights=(1, 1000), wseed=2) as g:
+        h = host_csr(g)
+        ref = oracle.sssp(h, 0)
+        d, _ = grus.sssp(g, 0)
+        assert (d == ref).all()
+        t0 = ctx.tuning
+        try:
+            for delta in (1, 333, 100000, 0xFFFFFFFF):
+                t = ctx.tuning
+                t.sssp_delta = delta
+                ctx.tuning = t
+                print("sssp grid delta", delta, flush=True)
+                d, _ = grus.sssp(g, 0)
+                assert (d == ref).all(), delta
+        finally:
+            ctx.tuning = t0
+        s = 256 * 160 + 77
+        assert (grus.sssp(g, s)[0] == oracle.sssp(h, s)).all()
+
+
+def test_sssp_unit_weights_equal_bfs(ctx, oracle):
+    with grus.Graph.rmat(ctx, 12) as g:  # unweighted: weight 1 per edge
+        s = g.info.max_degree_vertex
+        assert (grus.sssp(g, s)[0] == grus.bfs(g, s)[0]).all()
+    R, C = 40, 70
+    with grus.Graph.grid(ctx, R, C) as g:
+        rr, cc = np.divmod(np.arange(R * C), C)
+        assert (grus.sssp(g, 0)[0] == rr + cc).all()  # Manhattan distance
+
+
+def test_sssp_directed_weighted(ctx, oracle):
+    rng = np.random.default_rng(9)
+    off, col, w = random_directed(rng, 3000, 20000, wmax=500)
+    h = oracle.CSR(off, col, w)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=False) as g:
+        for s in (0, 11, 2999):
+            assert (grus.sssp(g, s)[0] == oracle.sssp(h, s)).all()
+
+
+def test_sssp_repeatable(ctx):
+    with grus.Graph.rmat(ctx, 14, weights=(1, 255), wseed=3) as g:
+        s = g.info.max_degree_vertex
+        a, _ = grus.sssp(g, s)
+        b, _ = grus.sssp(g, s)
+        assert (a == b).all()
+
+
+# ------------------------------------------------------------------ PageRank
+@pytest.mark.parametrize("scale", [10, 16])
+def test_pagerank_rmat_vs_fp64(ctx, oracle, scale):
+    with grus.Graph.rmat(ctx, scale) as g:
+        h = host_csr(g)
+        r, st = grus.pagerank(g, 0.85, 20)
+        ref = oracle.pagerank(h, 0.85, 20)
+        l1 = np.abs(r.astype(np.float64) - ref).sum()
+        print("pagerank s%d L1 %.3e" % (scale, l1), flush=True)
+        assert l1 <= PR_TOL
+        assert st.iterations == 20
+
+
+def test_pagerank_closed_forms(ctx):
+    # cycle (2-regular): r_k = (1 - a^(k+1)) / V at every vertex
+    n, a = 5000, 0.85
+    off = np.arange(0, 2 * n + 1, 2, dtype=np.uint64)
+    col = np.stack([(np.arange(n) - 1) % n, (np.arange(n) + 1) % n], 1)
+    col = np.sort(col, 1).reshape(-1).astype(np.uint32)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=True) as g:
+        for it in (0, 1, 7, 20):
+            r, _ = grus.pagerank(g, a, it)
+            assert np.allclose(r, (1 - a ** (it + 1)) / n, rtol=2e-6, atol=0), it
+
+
+def test_pagerank_hub_rows_span_windows(ctx, oracle):
+    # a star with a 50000-leaf centre: the centre's row spans ~25 windows
+    L = 50000
+    V = L + 3
+    rows = [list(range(1, L + 1))] + [[0]] * L + [[V - 1], [V - 2]]
+    off = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint64)
+    col = np.concatenate([np.array(r, np.uint32) for r in rows])
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=True) as g:
+        r, _ = grus.pagerank(g, 0.85, 20)
+        ref = oracle.pagerank(h, 0.85, 20)
+        assert np.abs(r - ref).sum() <= PR_TOL
+        assert abs(r[0] - ref[0]) / ref[0] < 1e-5
+
+
+def test_pagerank_directed_uses_out_degree(ctx, oracle):
+    rng = np.random.default_rng(4)
+    off, col, _ = random_directed(rng, 4000, 30000)
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=False) as g:
+        r, _ = grus.pagerank(g, 0.85, 20)
+        ref = oracle.pagerank(h, 0.85, 20, csc=oracle.transpose(h))
+        assert np.abs(r - ref).sum() <= PR_TOL
+
+
+def test_pagerank_bit_repeatable(ctx):
+    with grus.Graph.rmat(ctx, 16) as g:
+        a, _ = grus.pagerank(g)
+        b, _ = grus.pagerank(g)
+        assert (a.view(np.uint32) == b.view(np.uint32)).all()
+
+
+# ------------------------------------------------------------------ CC
+@pytest.mark.parametrize("scale", [10, 16])
+@pytest.mark.parametrize("method", ["hook", "lp"])
+def test_cc_rmat_vs_oracle(ctx, oracle, scale, method):
+    with grus.Graph.rmat(ctx, scale) as g:
+        h = host_csr(g)
+        lab, _ = grus.cc(g, method)
+        assert (lab == oracle.cc(h)).all()
+
+
+@pytest.mark.parametrize("method", ["hook", "lp"])
+def test_cc_many_components_and_grid(ctx, oracle, method):
+    rng = np.random.default_rng(2)
+    V = 20000
+    # sparse random symmetric graph: many small components
+    u = rng.integers(0, V, 8000)
+    v = rng.integers(0, V, 8000)
+    key = np.unique(np.concatenate([u.astype(np.uint64) << np.uint64(32) | v.astype(np.uint64),
+                                    v.astype(np.uint64) << np.uint64(32) | u.astype(np.uint64)]))
+    key = key[(key >> np.uint64(32)) != (key & np.uint64(0xFFFFFFFF))]
+    src = (key >> np.uint64(32)).astype(np.int64)
+    col = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    off = np.zeros(V + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=True) as g:
+        assert (grus.cc(g, method)[0] == oracle.cc(h)).all()
+    with grus.Graph.grid(ctx, 100, 130) as g:
+        assert (grus.cc(g, method)[0] == 0).all()
+
+
+def test_cc_directed_rejected(ctx):
+    off = np.array([0, 1, 1], np.uint64)
+    col = np.array([1], np.uint32)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=False) as g:
+        with pytest.raises(grus.GrusError) as e:
+            grus.cc(g)
+        assert e.value.code == capi.E_UNSUPPORTED
+
+
+# ------------------------------------------------------------------ larger
+def test_apps_rmat_s20_vs_oracle(ctx, oracle):
+    with grus.Graph.rmat(ctx, 20, weights=(1, 255), wseed=3) as g:
+        h = host_csr(g)
+        s = g.info.max_degree_vertex
+        print("s20 sssp", flush=True)
+        assert (grus.sssp(g, s)[0] == oracle.sssp(h, s)).all()
+        print("s20 cc", flush=True)
+        assert (grus.cc(g)[0] == oracle.cc(h)).all()
+        print("s20 pagerank", flush=True)
+        r, _ = grus.pagerank(g)
+        assert np.abs(r - oracle.pagerank(h)).sum() <= PR_TOL
