@@ -38,24 +38,36 @@ __global__ void k_queue_to_bitmap(const uint32_t *q, uint32_t n, uint64_t *bm) {
   }
 }
 
-__global__ void k_bitmap_to_queue(const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count) {
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  // uniform trip count per wave: iterate over wave-aligned bases
-  uint64_t first = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~(uint64_t)(WAVE - 1));
-  for (uint64_t base = first; base < W; base += stride) {
-    uint64_t wd = base + lane_id();
+__global__ __launch_bounds__(BLOCK) void k_bitmap_to_queue(const uint64_t *bm, uint64_t W,
+                                                            uint32_t *q, uint32_t *count) {
+  __shared__ uint32_t s_wave[BLOCK / WAVE + 1];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // block-uniform trip count; one lane per bitmap word
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < W; base += stride) {
+    uint64_t wd = base + threadIdx.x;
     uint64_t m = wd < W ? bm[wd] : 0ull;
     uint32_t c = (uint32_t)__popcll(m);
-    uint32_t x = c;
+    uint32_t x = c;  // inclusive wave scan of the popcounts
 #pragma unroll
     for (int o = 1; o < WAVE; o <<= 1) {
       uint32_t y = __shfl_up(x, o);
       if ((int)lane_id() >= o) x += y;
     }
-    uint32_t total = __shfl(x, WAVE - 1);
-    uint32_t at = 0;
-    if (lane_id() == 0 && total) at = atomicAdd(count, total);
-    at = __shfl(at, 0) + (x - c);
+    const uint32_t wid = threadIdx.x / WAVE;
+    if (lane_id() == WAVE - 1) s_wave[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (uint32_t i = 0; i < BLOCK / WAVE; i++) {
+        uint32_t t = s_wave[i];
+        s_wave[i] = s;
+        s += t;
+      }
+      s_wave[BLOCK / WAVE] = s ? atomicAdd(count, s) : 0u;
+    }
+    __syncthreads();
+    uint32_t at = s_wave[BLOCK / WAVE] + s_wave[wid] + (x - c);
+    __syncthreads();
     while (m) {
       int b = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;

@@ -83,6 +83,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
   __shared__ uint64_t sRow[LB_WIN];
   __shared__ uint32_t sQ[LB_WIN];
   __shared__ uint32_t s_i0, s_n;
+  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
   uint64_t my_edges = 0;
   for (uint64_t c0 = (uint64_t)blockIdx.x * LB_CHUNK; c0 < total;
        c0 += (uint64_t)gridDim.x * LB_CHUNK) {
@@ -131,13 +132,13 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
         bool upd = job.update(u, dst, e, edge_w(g, e));
         gen = job.generate(upd, dst);
       }
-      uint32_t slot = wave_append(&cnt->next_count, gen == 1);
+      uint32_t slot = block_append(&cnt->next_count, gen == 1, s_app);
       if (gen == 1) {
         qn[slot] = dst;
         my_edges += job.out_degree(dst);
       }
       if (Job::kFar) {
-        uint32_t fslot = wave_append(&cnt->far_count, gen == 2);
+        uint32_t fslot = block_append(&cnt->far_count, gen == 2, s_app);
         if (gen == 2) qfar[fslot] = dst;
       }
     }

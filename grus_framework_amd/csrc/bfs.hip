@@ -28,6 +28,8 @@
 //     Replaces Frontier<BDF_AUTO>'s fraction thresholds
 //     (include/frontier.cuh:140,206-215).
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 
 #include "advance.hpp"
 #include "apps.hpp"
@@ -92,7 +94,12 @@ __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uin
   }
 }
 
-// bottom-up step over every bitmap word (see file comment)
+// bottom-up step over every bitmap word (see file comment).  A wave takes
+// PULL_UNROLL consecutive words (256 vertices) per step and issues their
+// visited words, offsets, first in-neighbours and frontier bits together,
+// so each lane keeps PULL_UNROLL independent load chains in flight; most
+// RMAT vertices find a parent at their first (lowest-id) in-neighbour.
+constexpr int PULL_UNROLL = 4;
 __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_off,
                                                 const uint64_t *front, uint64_t *visited,
                                                 uint64_t *front_next, uint32_t *label,
@@ -103,38 +110,62 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
   uint32_t lane = lane_id();
   uint32_t my_found = 0;
   uint64_t my_edges = 0;
-  for (uint64_t wd = wave; wd < W; wd += nwaves) {
-    uint64_t vis = visited[wd];
-    if (vis == ~0ull) {
-      if (lane == 0) front_next[wd] = 0;
-      continue;
-    }
-    uint32_t v = (uint32_t)(wd * 64 + lane);
-    bool active = ((vis >> lane) & 1ull) == 0;  // pull_selector: label == INFINIT
-    bool found = false, is_long = false;
-    if (active) {
-      uint64_t e = in.off[v], end = in.off[v + 1];
-      uint64_t lim = (end - e > K) ? e + K : end;
-      for (; e < lim; e++) {
-        if (bm_test(front, edge_col(in, e))) {
-          found = true;
-          break;
-        }
+  for (uint64_t w0 = wave * PULL_UNROLL; w0 < W; w0 += nwaves * PULL_UNROLL) {
+    uint64_t vis[PULL_UNROLL], beg[PULL_UNROLL], end[PULL_UNROLL];
+    bool active[PULL_UNROLL], found[PULL_UNROLL];
+    uint32_t first[PULL_UNROLL];
+#pragma unroll
+    for (int j = 0; j < PULL_UNROLL; j++)
+      vis[j] = (w0 + j < W) ? visited[w0 + j] : ~0ull;
+#pragma unroll
+    for (int j = 0; j < PULL_UNROLL; j++) {
+      // pull_selector (bfs.cu:80-87): label == INFINIT, read as a clear visited bit
+      active[j] = ((vis[j] >> lane) & 1ull) == 0;
+      uint32_t v = (uint32_t)((w0 + j) * 64 + lane);
+      beg[j] = end[j] = 0;
+      if (active[j]) {
+        beg[j] = in.off[v];
+        end[j] = in.off[v + 1];
       }
-      is_long = !found && lim < end;
     }
-    uint64_t m = __ballot(found);
-    if (lane == 0) {
-      front_next[wd] = m;
-      if (m) visited[wd] = vis | m;
+#pragma unroll
+    for (int j = 0; j < PULL_UNROLL; j++)
+      first[j] = (beg[j] < end[j]) ? edge_col(in, beg[j]) : 0u;
+#pragma unroll
+    for (int j = 0; j < PULL_UNROLL; j++)
+      found[j] = (beg[j] < end[j]) && bm_test(front, first[j]);
+#pragma unroll
+    for (int j = 0; j < PULL_UNROLL; j++) {
+      if (vis[j] == ~0ull) {
+        if (lane == 0 && w0 + j < W) front_next[w0 + j] = 0;
+        continue;
+      }
+      uint32_t v = (uint32_t)((w0 + j) * 64 + lane);
+      bool is_long = false;
+      if (active[j] && !found[j] && beg[j] < end[j]) {
+        uint64_t e = beg[j] + 1;
+        uint64_t lim = (end[j] - beg[j] > K) ? beg[j] + K : end[j];
+        for (; e < lim; e++) {
+          if (bm_test(front, edge_col(in, e))) {
+            found[j] = true;
+            break;
+          }
+        }
+        is_long = !found[j] && lim < end[j];
+      }
+      uint64_t m = __ballot(found[j]);
+      if (lane == 0) {
+        front_next[w0 + j] = m;
+        if (m) visited[w0 + j] = vis[j] | m;
+      }
+      if (found[j]) {
+        label[v] = next_level;
+        my_found++;
+        my_edges += out_off[v + 1] - out_off[v];
+      }
+      uint32_t slot = wave_append(&cnt->long_count, is_long);
+      if (is_long) longq[slot] = v;
     }
-    if (found) {
-      label[v] = next_level;
-      my_found++;
-      my_edges += out_off[v + 1] - out_off[v];
-    }
-    uint32_t slot = wave_append(&cnt->long_count, is_long);
-    if (is_long) longq[slot] = v;
   }
   my_found = wave_sum(my_found);
   my_edges = wave_sum(my_edges);
@@ -226,6 +257,9 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
   uint64_t mu = g.E >= src_deg ? g.E - src_deg : 0;
   uint64_t v_reached = 1, e_reached = src_deg;
   uint32_t level = 0, pull_levels = 0;
+  // GRUS_TRACE=1 prints one line per level (mode, frontier, host time)
+  static const bool trace = getenv("GRUS_TRACE") != nullptr;
+  auto t_level = std::chrono::steady_clock::now();
   while (nf) {
     // every level discovers at least one new vertex, so V levels is a bound
     if (level > V) raise(GRUS_E_HIP, "bfs did not converge within V levels");
@@ -239,7 +273,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
         queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nf, w.front[cur].as<uint64_t>(), W);
         launches += 2;
       }
-      bfs::k_pull<<<grid_cap(ctx, W * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+      bfs::k_pull<<<grid_cap(ctx, div_up(W, bfs::PULL_UNROLL) * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
           in, out.off, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),
           w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K, w.longq.as<uint32_t>(), cnt, W);
       GRUS_LAUNCH_CHECK();
@@ -268,6 +302,14 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
     }
     ctx.read_ctl(2);  // next_count | long_count, next_edges
     const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
+    if (trace) {
+      auto now = std::chrono::steady_clock::now();
+      fprintf(stderr, "bfs level %u %s nf=%llu mf=%llu -> next %u (long %u) %.1f us\n", level,
+              pull ? "pull" : "push", (unsigned long long)nf, (unsigned long long)mf,
+              hc->next_count, hc->long_count,
+              std::chrono::duration<double, std::micro>(now - t_level).count());
+      t_level = now;
+    }
     prev_nf = nf;
     nf = hc->next_count;
     mf = hc->next_edges;

@@ -29,6 +29,31 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *count, bool pred) {
   return pred ? base + lane_rank(m) : 0xFFFFFFFFu;
 }
 
+// Workgroup-aggregated append: every thread of the workgroup must call it
+// (block-uniform control flow).  One atomic per workgroup; lanes get their
+// slot in wave order, lane order within a wave.  `lds` holds >= waves + 1
+// u32 of shared scratch.  Returns UINT32_MAX for threads with pred unset.
+__device__ __forceinline__ uint32_t block_append(uint32_t *count, bool pred, uint32_t *lds) {
+  const uint32_t nwaves = blockDim.x / WAVE;
+  const uint32_t wid = threadIdx.x / WAVE;
+  uint64_t m = __ballot(pred);
+  if (lane_id() == 0) lds[wid] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < nwaves; i++) {
+      uint32_t c = lds[i];
+      lds[i] = s;
+      s += c;
+    }
+    lds[nwaves] = s ? atomicAdd(count, s) : 0u;
+  }
+  __syncthreads();
+  uint32_t slot = lds[nwaves] + lds[wid] + lane_rank(m);
+  __syncthreads();  // lds reusable by the next call
+  return pred ? slot : 0xFFFFFFFFu;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll

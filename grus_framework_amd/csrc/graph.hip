@@ -68,6 +68,9 @@ Context::Context(int dev) {
   hipDeviceProp_t prop;
   GRUS_HIP(hipGetDeviceProperties(&prop, dev));
   cu_count = prop.multiProcessorCount;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+    wall_khz = khz;
   GRUS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   GRUS_HIP(hipEventCreate(&ev0));
   GRUS_HIP(hipEventCreate(&ev1));

@@ -57,6 +57,7 @@ struct Context {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int cu_count = 256;
+  int wall_khz = 100000;  // device wall clock rate (wall_clock64 ticks per ms)
   grus_tuning tuning{};
   uint64_t *h_ctl = nullptr;  // pinned host mirror of the control block
   Buf d_ctl;                  // device control block (counters)
@@ -67,6 +68,7 @@ struct Context {
   void activate() const { GRUS_HIP(hipSetDevice(device)); }
   // copy the first n u64 of the device control block to h_ctl and wait
   void read_ctl(size_t n);
+  uint64_t wall_ticks_per_ms() const { return (uint64_t)(wall_khz > 0 ? wall_khz : 100000); }
 };
 
 // POD view of a CSR (or CSC) handed to kernels.  Edges [0, split) are read

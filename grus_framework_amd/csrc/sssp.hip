@@ -14,14 +14,16 @@
 // queue is split.  Queue membership is deduplicated with per-vertex round
 // stamps (u32), which never need clearing between rounds.
 #include <algorithm>
+#include <cstdlib>
 
 #include "advance.hpp"
 #include "apps.hpp"
+#include "persist.hpp"
 
 namespace mgg {
 
 struct SsspWork {
-  Buf q[2], far[2], near_stamp, far_stamp;
+  Buf q[2], far[2], near_stamp, far_stamp, pctl;
   PushWork push;
   uint64_t cap_v = 0;
   uint32_t stamp = 0;  // last stamp handed out; stamps only grow
@@ -150,6 +152,172 @@ __global__ void k_weight_sum(const uint32_t *w, uint64_t E, unsigned long long *
   if (lane_id() == 0) atomicAdd(out, s);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent near-far for low-degree graphs (grids, road-like inputs): the
+// whole traversal is one cooperative launch; rounds are separated by
+// grid_sync instead of a return to the host.  A lane owns one frontier
+// vertex and walks its (at most max_deg) edges; generated vertices are
+// appended with one atomic per workgroup.  Counters live in a ring of 4
+// slots: slot s&3 belongs to step s, workgroup 0 clears slot (s+2)&3 at the
+// start of step s (no workgroup can still be reading it), and every
+// workgroup derives identical loop state from the slots it reads after a
+// barrier.
+struct PCtl {
+  GridSync gs;
+  uint32_t ring[4][4];  // per step: next count, far count, min far distance, unused
+  uint32_t rounds, stamp, status, pad;
+};
+
+struct PParams {
+  CsrView g;
+  uint32_t *dist, *q0, *q1, *far0, *far1, *near_stamp, *far_stamp;
+  PCtl *ctl;
+  uint64_t delta, round_cap, timeout;
+  uint32_t stamp0, max_deg;
+};
+
+__device__ __forceinline__ void clear_slot(PCtl *c, uint32_t step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t *z = c->ring[(step + 2) & 3];
+    st_agent(&z[0], 0);
+    st_agent(&z[1], 0);
+    st_agent(&z[2], 0xFFFFFFFFu);
+  }
+}
+
+// LDS staging of one round's output: vertices generated by a workgroup are
+// collected in LDS and published with one atomic per workgroup per round
+constexpr uint32_t PQ_CAP = 2048;
+
+__device__ __forceinline__ void stage(uint32_t v, uint32_t *lq, uint32_t *lcount,
+                                      uint32_t *gcount, uint32_t *gq, uint32_t goff) {
+  uint32_t s = atomicAdd(lcount, 1u);
+  if (s < PQ_CAP) {
+    lq[s] = v;
+  } else {  // overflow: publish directly
+    uint32_t g = atomicAdd(gcount, 1u);
+    st_agent(&gq[goff + g], v);
+  }
+}
+
+__device__ __forceinline__ void publish(const uint32_t *lq, uint32_t lcount, uint32_t *gcount,
+                                        uint32_t *gq, uint32_t goff, uint32_t *s_base) {
+  uint32_t n = lcount < PQ_CAP ? lcount : PQ_CAP;
+  if (threadIdx.x == 0) *s_base = n ? atomicAdd(gcount, n) : 0u;
+  __syncthreads();
+  const uint32_t base = *s_base;
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) st_agent(&gq[goff + base + k], lq[k]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
+  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP];
+  __shared__ uint32_t s_nn, s_nf, s_base;
+  const CsrView g = p.g;
+  unsigned int gen = 0;
+  uint32_t step = 0, stamp = p.stamp0;
+  uint32_t far_id = ++stamp;
+  uint32_t nn = 1, nfar = 0;
+  uint64_t T = p.delta, rounds = 0;
+  uint32_t *q[2] = {p.q0, p.q1};
+  uint32_t *far[2] = {p.far0, p.far1};
+  int cur = 0, fcur = 0;
+  bool ok = true;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  while (ok) {
+    uint32_t *C = p.ctl->ring[step & 3];
+    clear_slot(p.ctl, step);
+    if (nn > 0) {
+      if (++rounds > p.round_cap) {
+        ok = false;
+        break;
+      }
+      const uint32_t near_id = ++stamp;
+      const uint32_t Tk = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
+      if (threadIdx.x == 0) s_nn = s_nf = 0;
+      __syncthreads();
+      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride) {
+        const uint32_t u = ld_agent(&q[cur][i]);
+        const uint64_t e0 = g.off[u], e1 = g.off[u + 1];
+        const uint32_t du = ld_agent(&p.dist[u]);
+        for (uint64_t e = e0; e < e1; e++) {
+          const uint32_t v = edge_col(g, e);
+          const uint64_t nd = (uint64_t)du + edge_w(g, e);
+          if (nd >= INFINIT || atomicMin(&p.dist[v], (uint32_t)nd) <= (uint32_t)nd) continue;
+          // generator: near below T, far otherwise, once per queue (stamps)
+          if (nd < Tk) {
+            if (atomicExch(&p.near_stamp[v], near_id) != near_id)
+              stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
+          } else if (atomicExch(&p.far_stamp[v], far_id) != far_id) {
+            stage(v, s_far, &s_nf, &C[1], far[fcur], nfar);
+          }
+        }
+      }
+      __syncthreads();
+      publish(s_near, s_nn, &C[0], q[cur ^ 1], 0, &s_base);
+      __syncthreads();
+      publish(s_far, s_nf, &C[1], far[fcur], nfar, &s_base);
+      ok = grid_sync_atomic(&p.ctl->gs, gridDim.x, gen, p.timeout);
+      step++;
+      nn = ld_agent(&C[0]);
+      nfar += ld_agent(&C[1]);
+      cur ^= 1;
+    } else if (nfar > 0) {
+      // advance T to the bucket of the smallest live far distance
+      const uint32_t Tk = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
+      uint32_t m = 0xFFFFFFFFu;
+      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nfar; i += stride) {
+        uint32_t d = ld_agent(&p.dist[ld_agent(&far[fcur][i])]);
+        if (d >= Tk && d < m) m = d;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        uint32_t x = __shfl_xor(m, o);
+        m = x < m ? x : m;
+      }
+      if (lane_id() == 0 && m != 0xFFFFFFFFu) atomicMin(&C[2], m);
+      ok = grid_sync_atomic(&p.ctl->gs, gridDim.x, gen, p.timeout);
+      step++;
+      const uint32_t mn = ld_agent(&C[2]);
+      if (!ok || mn == 0xFFFFFFFFu) break;  // every far entry was stale
+      const uint64_t Tn = ((uint64_t)mn / p.delta + 1) * p.delta;
+      const uint32_t Tnk = Tn < 0xFFFFFFFFull ? (uint32_t)Tn : 0xFFFFFFFFu;
+      far_id = ++stamp;
+      const uint32_t near_id = ++stamp;
+      uint32_t *C2 = p.ctl->ring[step & 3];
+      clear_slot(p.ctl, step);
+      if (threadIdx.x == 0) s_nn = s_nf = 0;
+      __syncthreads();
+      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nfar; i += stride) {
+        const uint32_t v = ld_agent(&far[fcur][i]);
+        const uint32_t d = ld_agent(&p.dist[v]);
+        if (d < Tk) continue;  // relaxed in an earlier bucket at its final distance
+        if (d < Tnk) {
+          if (atomicExch(&p.near_stamp[v], near_id) != near_id)
+            stage(v, s_near, &s_nn, &C2[0], q[cur], 0);
+        } else if (atomicExch(&p.far_stamp[v], far_id) != far_id) {
+          stage(v, s_far, &s_nf, &C2[1], far[fcur ^ 1], 0);
+        }
+      }
+      __syncthreads();
+      publish(s_near, s_nn, &C2[0], q[cur], 0, &s_base);
+      __syncthreads();
+      publish(s_far, s_nf, &C2[1], far[fcur ^ 1], 0, &s_base);
+      ok = grid_sync_atomic(&p.ctl->gs, gridDim.x, gen, p.timeout);
+      step++;
+      nn = ld_agent(&C2[0]);
+      nfar = ld_agent(&C2[1]);
+      fcur ^= 1;
+      T = Tn;
+    } else {
+      break;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p.ctl->rounds = (uint32_t)rounds;
+    p.ctl->stamp = stamp;
+    p.ctl->status = ok ? 0u : 1u;
+  }
+}
+
 }  // namespace sssp
 
 namespace {
@@ -178,6 +346,56 @@ uint32_t pick_delta(Graph &g, SsspWork &w) {
 }
 }  // namespace
 
+namespace {
+// one cooperative launch for the whole traversal (low max degree)
+void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, uint64_t *rounds) {
+  Context &ctx = *g.ctx;
+  const uint64_t V = g.V;
+  if (w.stamp >= 0xE0000000u) {
+    fill_u32(ctx, w.near_stamp.as<uint32_t>(), V + 1, 0);
+    fill_u32(ctx, w.far_stamp.as<uint32_t>(), V + 1, 0);
+    w.stamp = 0;
+  }
+  w.pctl.ensure(sizeof(sssp::PCtl));
+  sssp::PCtl init{};
+  for (auto &slot : init.ring) slot[2] = 0xFFFFFFFFu;
+  GRUS_HIP(hipMemcpyAsync(w.pctl.p, &init, sizeof init, hipMemcpyHostToDevice, ctx.stream));
+  sssp::PParams p{};
+  p.g = g.csr();
+  p.dist = g.result.as<uint32_t>();
+  p.q0 = w.q[0].as<uint32_t>();
+  p.q1 = w.q[1].as<uint32_t>();
+  p.far0 = w.far[0].as<uint32_t>();
+  p.far1 = w.far[1].as<uint32_t>();
+  p.near_stamp = w.near_stamp.as<uint32_t>();
+  p.far_stamp = w.far_stamp.as<uint32_t>();
+  p.ctl = w.pctl.as<sssp::PCtl>();
+  p.delta = delta;
+  p.round_cap = std::min<uint64_t>(64 * V + 1024, 0x08000000ull);
+  p.timeout = ctx.wall_ticks_per_ms() * 20000;  // 20 s per barrier wait
+  p.stamp0 = w.stamp;
+  p.max_deg = (uint32_t)std::max<uint64_t>(g.max_deg, 1);
+  int per_cu = 0;
+  GRUS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(sssp::k_persistent), BLOCK, 0));
+  if (per_cu < 1) raise(GRUS_E_HIP, "persistent sssp kernel cannot be resident");
+  // few workgroups keep the barrier cheap; the frontier of a high-diameter
+  // graph is small (GRUS_PERSIST_BLOCKS overrides for experiments)
+  static const char *env_blocks = getenv("GRUS_PERSIST_BLOCKS");
+  unsigned want = env_blocks ? (unsigned)atoi(env_blocks) : (unsigned)std::max(1, ctx.cu_count / 8);
+  unsigned grid = std::max(1u, std::min(want, (unsigned)(ctx.cu_count * per_cu)));
+  void *args[] = {&p};
+  GRUS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sssp::k_persistent),
+                                      dim3(grid), dim3(BLOCK), args, 0, ctx.stream));
+  sssp::PCtl out;
+  GRUS_HIP(hipMemcpyAsync(&out, w.pctl.p, sizeof out, hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  w.stamp = out.stamp;
+  *rounds = out.rounds;
+  if (out.status) raise(GRUS_E_TIMEOUT, "persistent sssp aborted (barrier timeout or round cap)");
+}
+}  // namespace
+
 void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   Context &ctx = *g.ctx;
   ctx.activate();
@@ -200,13 +418,20 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     src_deg = o[1] - o[0];
   }
   uint64_t launches = 0, rounds = 0, relaxed_vertices = 0;
+  // low-degree graphs (grids, road-like): one persistent cooperative launch
+  static const bool no_persist = getenv("GRUS_NO_PERSISTENT") != nullptr;
+  const bool persistent = g.max_deg <= 32 && !no_persist;
   GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   sssp::k_init<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(dist, V, src,
                                                                  w.q[0].as<uint32_t>());
   GRUS_LAUNCH_CHECK();
   launches++;
+  if (persistent) {
+    run_sssp_persistent(g, w, src, delta, &rounds);
+    launches++;
+  }
   int cur = 0, fcur = 0;
-  uint64_t nn = 1, mf = src_deg, nfar = 0;
+  uint64_t nn = persistent ? 0 : 1, mf = src_deg, nfar = 0;
   uint64_t T = delta;
   uint32_t far_id = w.next_stamp(ctx, V);
   const uint64_t round_cap = 64 * V + 1024;
