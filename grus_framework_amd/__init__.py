@@ -159,6 +159,12 @@ class Graph:
         capi.check(_lib().grus_result_copy(self.h, _p(out)))
         return out
 
+    def digest(self):
+        """(reached, checksum) of the last u32 result (grus_result_digest)."""
+        r, c = C.c_uint64(), C.c_uint64()
+        capi.check(_lib().grus_result_digest(self.h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
     def close(self):
         if self.h:
             capi.check(_lib().grus_graph_destroy(self.h))

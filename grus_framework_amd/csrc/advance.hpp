@@ -148,14 +148,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
   if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
 }
 
-// Host side of the push operator: degree scan + edge-balanced expansion.
-struct PushWork {
-  Buf deg, scan;
-  void ensure(uint64_t V) {
-    deg.ensure((V + 2) * 8);
-    scan.ensure((V + 2) * 8);
-  }
-};
+// Host side of the push operator: degree scan + edge-balanced expansion
+// (PushWork, the scan buffers, is declared in graph.hpp).
 
 // w.scan[0..F] <- exclusive scan of outdeg(q[i]); w.scan[F] = total
 void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uint32_t *q,

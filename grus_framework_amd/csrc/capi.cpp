@@ -10,6 +10,7 @@
 
 #include "apps.hpp"
 #include "graph.hpp"
+#include "multi.hpp"
 #include "serve.hpp"
 
 struct grus_ctx {
@@ -270,6 +271,17 @@ int grus_result_copy(grus_graph *g, void *out) {
     need(out, "out");
     if (!g->last_result) mgg::raise(GRUS_E_INVALID, "no result held for this graph");
     copy_out(g, out);
+  });
+}
+
+int grus_result_digest(grus_graph *g, uint64_t *reached, uint64_t *checksum) {
+  return guard([&] {
+    need(g, "g");
+    if (g->last_result != 1) mgg::raise(GRUS_E_INVALID, "no u32 result held for this graph");
+    uint64_t r = 0, c = 0;
+    mgg::row_digest(*g->g->ctx, g->g->result.as<uint32_t>(), g->g->V, &r, &c);
+    if (reached) *reached = r;
+    if (checksum) *checksum = c;
   });
 }
 

@@ -52,6 +52,15 @@ struct Buf {
   }
 };
 
+// scan buffers of the edge-balanced push operator (advance.hpp)
+struct PushWork {
+  Buf deg, scan;
+  void ensure(uint64_t V) {
+    deg.ensure((V + 2) * 8);
+    scan.ensure((V + 2) * 8);
+  }
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -88,7 +97,7 @@ struct BfsWork;
 struct SsspWork;
 struct PrWork;
 struct CcWork;
-struct MsWork;
+class MsEngine;
 
 struct Graph {
   Context *ctx = nullptr;
@@ -114,7 +123,7 @@ struct Graph {
   std::shared_ptr<SsspWork> sssp;
   std::shared_ptr<PrWork> pr;
   std::shared_ptr<CcWork> cc;
-  std::shared_ptr<MsWork> ms;
+  std::shared_ptr<MsEngine> ms;
 
   explicit Graph(Context *c);
   ~Graph();

@@ -1,0 +1,609 @@
+// multi.hip -- batched multi-source BFS and SSSP engines (see multi.hpp).
+//
+// BFS lanes: visit/front/next are u64 masks per vertex.  A step expands the
+// frontier either by push (edge-balanced operator over the queue of frontier
+// vertices: next[v] |= front[u] & ~visit[v]) or by pull (a lane per vertex
+// ORs front[] over its in-edges until every lane still missing v is
+// satisfied), then a finalize pass commits the new bits, writes the levels
+// (row per lane, coalesced: the wave writes one row at a time over 64
+// consecutive vertices) and rebuilds the frontier queue.
+// SSSP lanes: distances are stored vertex-major, dist[v*64 + lane], so one
+// relaxation of edge (u,v) for all lanes is a single coalesced 64-lane
+// atomicMin over 256 bytes.  Rounds are Bellman-Ford over the frontier
+// (front bit = the lane's distance at u improved last round); a wave owns a
+// chunk of up to 64 edges of one frontier vertex.
+#include <algorithm>
+
+#include "advance.hpp"
+#include "device.hpp"
+#include "multi.hpp"
+#include "prims.hpp"
+
+namespace mgg {
+
+namespace ms {
+
+constexpr uint32_t K_PULL = 32;
+constexpr uint32_t CHUNK = 64;
+
+struct LaneStart {
+  uint32_t s[64];
+};
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+  return x;
+}
+
+// order-independent digest of a result row: sum of mix64(v<<32 | value)
+__device__ __forceinline__ uint64_t digest_term(uint64_t v, uint32_t val) {
+  return mix64((v << 32) | val);
+}
+
+__global__ void k_reset_masks(uint64_t *visit, uint64_t *front, uint64_t *next, uint64_t V,
+                              uint64_t keep) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V; i += stride) {
+    if (visit) visit[i] &= keep;
+    front[i] &= keep;
+    next[i] &= keep;
+  }
+}
+
+// BFS rows (row-major [lane][V]) of lanes in mask <- INFINIT
+__global__ void k_reset_rows(uint32_t *level, uint64_t V, uint64_t mask) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V; i += stride) {
+    uint64_t m = mask;
+    while (m) {
+      int b = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      level[(uint64_t)b * V + i] = INFINIT;
+    }
+  }
+}
+
+// SSSP columns (vertex-major [V][64]) of lanes in mask <- INFINIT
+__global__ void k_reset_cols(uint32_t *dist, uint64_t V, uint64_t mask) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V * 64; i += stride)
+    if ((mask >> (i & 63)) & 1ull) dist[i] = INFINIT;
+}
+
+__global__ void k_inject(const uint32_t *srcs, const uint32_t *lanes, int n, bool bfs,
+                         uint64_t *visit, uint64_t *front, uint32_t *vals, uint64_t V,
+                         uint32_t *queue, MsCounters *cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t s = srcs[k], lane = lanes[k];
+  unsigned long long bit = 1ull << lane;
+  if (bfs) {
+    atomicOr((unsigned long long *)&visit[s], bit);
+    vals[(uint64_t)lane * V + s] = 0;  // level[lane][s]
+  } else {
+    vals[(uint64_t)s * 64 + lane] = 0;  // dist[s][lane]
+  }
+  unsigned long long old = atomicOr((unsigned long long *)&front[s], bit);
+  if (old == 0) queue[atomicAdd(&cnt->qcount, 1u)] = s;  // not yet queued
+}
+
+// ---- BFS pull ----
+__global__ __launch_bounds__(BLOCK) void k_bfs_pull(CsrView in, const uint64_t *visit,
+                                                    const uint64_t *front, uint64_t *next,
+                                                    uint64_t running, uint32_t *longq,
+                                                    MsCounters *cnt) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t first = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~(uint64_t)(WAVE - 1));
+  for (uint64_t base = first; base < in.V; base += stride) {
+    uint64_t v = base + lane_id();
+    bool is_long = false;
+    if (v < in.V) {
+      uint64_t need = running & ~visit[v];
+      if (need) {
+        uint64_t e = in.off[v], e1 = in.off[v + 1];
+        uint64_t lim = (e1 - e > K_PULL) ? e + K_PULL : e1;
+        uint64_t acc = 0;
+        for (; e < lim; e++) {
+          acc |= front[edge_col(in, e)];
+          if ((acc & need) == need) break;
+        }
+        acc &= need;
+        is_long = (acc != need) && lim < e1;
+        if (acc) next[v] = acc;
+      }
+    }
+    uint32_t slot = wave_append(&cnt->long_count, is_long);
+    if (is_long) longq[slot] = (uint32_t)v;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bfs_pull_long(CsrView in, const uint64_t *visit,
+                                                         const uint64_t *front, uint64_t *next,
+                                                         uint64_t running, const uint32_t *longq,
+                                                         const MsCounters *cnt) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint32_t n = cnt->long_count;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    uint32_t v = longq[i];
+    uint64_t need = running & ~visit[v];
+    uint64_t acc = next[v];
+    uint64_t e1 = in.off[v + 1];
+    for (uint64_t base = in.off[v] + K_PULL; base < e1 && (acc & need) != need; base += WAVE) {
+      uint64_t e = base + lane_id();
+      uint64_t x = e < e1 ? front[edge_col(in, e)] : 0ull;
+      acc |= wave_or(x);
+    }
+    if (lane_id() == 0) next[v] = acc & need;
+  }
+}
+
+// ---- BFS push job for the edge-balanced operator ----
+struct push_job_t {
+  static constexpr bool kFar = false;
+  const uint64_t *front, *visit;
+  uint64_t *next;
+  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t) const {
+    uint64_t m = front[src] & ~visit[dst];
+    if (m && (next[dst] & m) != m) atomicOr((unsigned long long *)&next[dst], m);
+    return false;
+  }
+  __device__ __forceinline__ int generate(bool, vtx_t) const { return 0; }
+  __device__ __forceinline__ uint64_t out_degree(vtx_t) const { return 0; }
+};
+
+// ---- BFS finalize: commit new bits, levels, queue, lane counts ----
+__global__ __launch_bounds__(BLOCK) void k_bfs_finalize(const uint64_t *off, uint64_t *visit,
+                                                        uint64_t *front, uint64_t *next,
+                                                        uint32_t *level, uint64_t V,
+                                                        uint64_t running, uint32_t t,
+                                                        LaneStart st, uint32_t *queue,
+                                                        MsCounters *cnt) {
+  __shared__ uint32_t s_cnt[64];
+  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+  uint64_t edges = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V; base += stride) {
+    uint64_t v = base + threadIdx.x;
+    uint64_t nw = 0;
+    if (v < V) {
+      uint64_t nx = next[v];
+      if (nx) {
+        next[v] = 0;
+        uint64_t vis = visit[v];
+        nw = nx & ~vis & running;
+        if (nw) visit[v] = vis | nw;
+      }
+      if (front[v] != nw) front[v] = nw;
+    }
+    uint64_t wm = wave_or(nw);
+    while (wm) {
+      int i = __ffsll((unsigned long long)wm) - 1;
+      wm &= wm - 1;
+      bool has = (nw >> i) & 1ull;
+      if (has) level[(uint64_t)i * V + v] = t + 1 - st.s[i];
+      uint64_t b = __ballot(has);
+      if (lane_id() == 0) atomicAdd(&s_cnt[i], (uint32_t)__popcll(b));
+    }
+    uint32_t slot = block_append(&cnt->qcount, nw != 0, s_app);
+    if (nw) {
+      queue[slot] = (uint32_t)v;
+      edges += off[v + 1] - off[v];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64; i += blockDim.x)
+    if (s_cnt[i]) atomicAdd(&cnt->lane_new[i], s_cnt[i]);
+  edges = wave_sum(edges);
+  if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
+}
+
+// ---- SSSP ----
+// chunk counts: ceil(deg(q[i]) / CHUNK), plus a trailing zero
+__global__ void k_chunk_counts(const uint64_t *off, const uint32_t *q, uint32_t F, uint64_t *c) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i <= F; i += stride) {
+    uint64_t d = 0;
+    if (i < F) {
+      uint32_t u = q[i];
+      d = (off[u + 1] - off[u] + CHUNK - 1) / CHUNK;
+    }
+    c[i] = d;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t *q, uint32_t F,
+                                                      const uint64_t *CS, uint64_t chunks,
+                                                      const uint64_t *front, uint64_t *next,
+                                                      uint32_t *dist) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint32_t lane = lane_id();
+  for (uint64_t c = wave; c < chunks; c += nwaves) {
+    uint32_t j = upper_bound_u64(CS, 0, F + 1, c) - 1;
+    uint32_t u = q[j];
+    uint64_t e = g.off[u] + (c - CS[j]) * CHUNK;
+    uint64_t e1 = g.off[u + 1];
+    if (e1 > e + CHUNK) e1 = e + CHUNK;
+    const uint64_t fm = front[u];
+    const bool act = (fm >> lane) & 1ull;
+    const uint32_t du = act ? dist[(uint64_t)u * 64 + lane] : INFINIT;
+    for (; e < e1; e++) {
+      uint32_t v = edge_col(g, e);
+      uint64_t nd = (uint64_t)du + edge_w(g, e);
+      bool imp = false;
+      if (act && nd < INFINIT) imp = atomicMin(&dist[(uint64_t)v * 64 + lane], (uint32_t)nd) > nd;
+      uint64_t m = __ballot(imp);
+      if (m && lane == 0) atomicOr((unsigned long long *)&next[v], m);
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, uint64_t *front,
+                                                         uint64_t *next, uint64_t V,
+                                                         uint64_t running, uint32_t *queue,
+                                                         MsCounters *cnt) {
+  __shared__ uint32_t s_cnt[64];
+  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+  uint64_t edges = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V; base += stride) {
+    uint64_t v = base + threadIdx.x;
+    uint64_t nw = 0;
+    if (v < V) {
+      uint64_t nx = next[v];
+      if (nx) next[v] = 0;
+      nw = nx & running;
+      if (front[v] != nw) front[v] = nw;
+    }
+    uint64_t wm = wave_or(nw);
+    while (wm) {
+      int i = __ffsll((unsigned long long)wm) - 1;
+      wm &= wm - 1;
+      uint64_t b = __ballot((nw >> i) & 1ull);
+      if (lane_id() == 0) atomicAdd(&s_cnt[i], (uint32_t)__popcll(b));
+    }
+    uint32_t slot = block_append(&cnt->qcount, nw != 0, s_app);
+    if (nw) {
+      queue[slot] = (uint32_t)v;
+      edges += off[v + 1] - off[v];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64; i += blockDim.x)
+    if (s_cnt[i]) atomicAdd(&cnt->lane_new[i], s_cnt[i]);
+  edges = wave_sum(edges);
+  if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
+}
+
+// ---- extraction ----
+__global__ void k_gather_col(const uint32_t *dist, uint64_t V, uint32_t lane, uint32_t *row) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V; i += stride) row[i] = dist[i * 64 + lane];
+}
+
+__global__ void k_digest(const uint32_t *row, uint64_t V, unsigned long long *out) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long h = 0, n = 0;
+  for (; i < V; i += stride) {
+    uint32_t x = row[i];
+    h += digest_term(i, x);
+    n += (x != INFINIT);
+  }
+  h = wave_sum(h);
+  n = wave_sum(n);
+  if (lane_id() == 0) {
+    atomicAdd(&out[0], h);
+    atomicAdd(&out[1], n);
+  }
+}
+
+}  // namespace ms
+
+void row_digest(Context &ctx, const uint32_t *row, uint64_t V, uint64_t *reached,
+                uint64_t *checksum) {
+  auto *d = reinterpret_cast<unsigned long long *>(ctx.d_ctl.p) + 48;
+  GRUS_HIP(hipMemsetAsync(d, 0, 16, ctx.stream));
+  ms::k_digest<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(row, V, d);
+  GRUS_LAUNCH_CHECK();
+  ctx.read_ctl(50);
+  *checksum = ctx.h_ctl[48];
+  *reached = ctx.h_ctl[49];
+}
+
+// ------------------------------------------------------------------ engine
+MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
+    : g_(g), kind_(kind), lanes_(std::max(1, std::min(lanes, 64))) {
+  if (own_stream) {
+    own_ctx_ = std::make_unique<Context>(g.ctx->device);
+    own_ctx_->tuning = g.ctx->tuning;
+    ctx_ = own_ctx_.get();
+  } else {
+    ctx_ = g.ctx;
+  }
+  Context &ctx = *ctx_;
+  ctx.activate();
+  const uint64_t V = g.V;
+  if (kind_ == MsKind::BFS) visit_.alloc((V + 1) * 8);
+  front_.alloc((V + 1) * 8);
+  next_.alloc((V + 1) * 8);
+  queue_.alloc((V + 64) * 4);
+  longq_.alloc((V + 1) * 4);
+  row_.alloc((V + 1) * 4);
+  digest_.alloc(64);
+  srcs_dev_.alloc(1024);
+  cnt_.alloc(sizeof(MsCounters));
+  push_cnt_.alloc(sizeof(StepCounters));
+  if (kind_ == MsKind::BFS) {
+    vals_.alloc((uint64_t)lanes_ * V * 4 + 8);  // level[lane][v]
+    fill_u64(ctx, visit_.as<uint64_t>(), V + 1, 0);
+  } else {
+    vals_.alloc(V * 64 * 4 + 8);  // dist[v][lane]
+  }
+  fill_u64(ctx, front_.as<uint64_t>(), V + 1, 0);
+  fill_u64(ctx, next_.as<uint64_t>(), V + 1, 0);
+  GRUS_HIP(hipMemsetAsync(cnt_.p, 0, sizeof(MsCounters), ctx.stream));
+  push_.ensure(V);
+  dirty_ = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);  // rows start uninitialised
+  ctx.sync();
+}
+
+MsEngine::~MsEngine() {
+  if (ctx_) (void)hipStreamSynchronize(ctx_->stream);
+}
+
+uint64_t MsEngine::state_bytes() const {
+  return visit_.bytes + front_.bytes + next_.bytes + vals_.bytes + queue_.bytes + longq_.bytes +
+         row_.bytes;
+}
+
+int MsEngine::free_lanes() const {
+  uint64_t all = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);
+  return __builtin_popcountll(all & ~busy_);
+}
+
+void MsEngine::reset_lanes(uint64_t mask) {
+  if (!mask) return;
+  Context &ctx = *ctx_;
+  const uint64_t V = g_.V;
+  ms::k_reset_masks<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+      kind_ == MsKind::BFS ? visit_.as<uint64_t>() : nullptr, front_.as<uint64_t>(),
+      next_.as<uint64_t>(), V, ~mask);
+  GRUS_LAUNCH_CHECK();
+  if (kind_ == MsKind::BFS) {
+    ms::k_reset_rows<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V,
+                                                                       mask);
+  } else {
+    ms::k_reset_cols<<<grid_cap(ctx, V * 64, BLOCK), BLOCK, 0, ctx.stream>>>(
+        vals_.as<uint32_t>(), V, mask);
+  }
+  GRUS_LAUNCH_CHECK();
+}
+
+std::vector<int> MsEngine::admit(const uint32_t *srcs, int n) {
+  Context &ctx = *ctx_;
+  ctx.activate();
+  std::vector<int> used;
+  uint64_t all = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);
+  uint64_t freem = all & ~busy_;
+  uint32_t hs[128];  // srcs then lanes
+  for (int k = 0; k < n && freem; k++) {
+    if (srcs[k] >= g_.V) raise(GRUS_E_INVALID, "source vertex out of range");
+    int lane = __builtin_ctzll(freem);
+    freem &= freem - 1;
+    used.push_back(lane);
+  }
+  const int m = (int)used.size();
+  if (!m) return used;
+  uint64_t mask = 0;
+  for (int k = 0; k < m; k++) {
+    hs[k] = srcs[k];
+    hs[64 + k] = (uint32_t)used[k];
+    mask |= 1ull << used[k];
+    start_[used[k]] = (uint32_t)t_;
+  }
+  reset_lanes(mask & dirty_);
+  dirty_ &= ~mask;
+  // frontier queue currently holds qn_ entries; injected sources are appended
+  auto *cnt = cnt_.as<MsCounters>();
+  MsCounters zero{};
+  zero.qcount = (uint32_t)qn_;
+  GRUS_HIP(hipMemcpyAsync(cnt, &zero, sizeof zero, hipMemcpyHostToDevice, ctx.stream));
+  GRUS_HIP(hipMemcpyAsync(srcs_dev_.p, hs, sizeof hs, hipMemcpyHostToDevice, ctx.stream));
+  ms::k_inject<<<1, 64, 0, ctx.stream>>>(srcs_dev_.as<uint32_t>(), srcs_dev_.as<uint32_t>() + 64,
+                                         m, kind_ == MsKind::BFS, visit_.as<uint64_t>(),
+                                         front_.as<uint64_t>(), vals_.as<uint32_t>(), g_.V,
+                                         queue_.as<uint32_t>(), cnt);
+  GRUS_LAUNCH_CHECK();
+  MsCounters h;
+  GRUS_HIP(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  qn_ = h.qcount;
+  busy_ |= mask;
+  return used;
+}
+
+std::vector<int> MsEngine::step() {
+  Context &ctx = *ctx_;
+  ctx.activate();
+  std::vector<int> done;
+  if (!busy_) return done;
+  const uint64_t V = g_.V;
+  const CsrView out = g_.csr();
+  auto *cnt = cnt_.as<MsCounters>();
+  // running lanes: busy lanes whose frontier was not empty (all busy lanes
+  // have a non-empty frontier: finished lanes are reported and leave the
+  // running set at once)
+  const uint64_t running = busy_ & ~finished_mask_;
+  GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(MsCounters), ctx.stream));
+  if (kind_ == MsKind::BFS) {
+    const CsrView in = g_.csc();
+    const bool pull = (double)front_edges_ > (double)g_.E / 24.0;
+    if (pull) {
+      ms::k_bfs_pull<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+          in, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), running,
+          longq_.as<uint32_t>(), cnt);
+      GRUS_LAUNCH_CHECK();
+      ms::k_bfs_pull_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+          in, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), running,
+          longq_.as<uint32_t>(), cnt);
+      GRUS_LAUNCH_CHECK();
+    } else if (qn_) {
+      push_scan_degrees(ctx, push_, out.off, queue_.as<uint32_t>(), (uint32_t)qn_);
+      uint64_t total = 0;
+      GRUS_HIP(hipMemcpyAsync(&total, push_.scan.as<uint64_t>() + qn_, 8, hipMemcpyDeviceToHost,
+                              ctx.stream));
+      ctx.sync();
+      ms::push_job_t job{front_.as<uint64_t>(), visit_.as<uint64_t>(), next_.as<uint64_t>()};
+      push_advance(ctx, push_, out, queue_.as<uint32_t>(), (uint32_t)qn_, total, job,
+                   nullptr, push_cnt_.as<StepCounters>());
+    }
+    ms::LaneStart st;
+    for (int i = 0; i < 64; i++) st.s[i] = start_[i];
+    GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(MsCounters), ctx.stream));
+    ms::k_bfs_finalize<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+        out.off, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(),
+        vals_.as<uint32_t>(), V, running, (uint32_t)t_, st, queue_.as<uint32_t>(), cnt);
+    GRUS_LAUNCH_CHECK();
+  } else {
+    if (qn_) {
+      Buf &cs = push_.scan;
+      ms::k_chunk_counts<<<grid_cap(ctx, qn_ + 1, BLOCK), BLOCK, 0, ctx.stream>>>(
+          out.off, queue_.as<uint32_t>(), (uint32_t)qn_, push_.deg.as<uint64_t>());
+      GRUS_LAUNCH_CHECK();
+      scan_excl_u64(ctx, push_.deg.as<uint64_t>(), cs.as<uint64_t>(), qn_ + 1);
+      uint64_t chunks = 0;
+      GRUS_HIP(hipMemcpyAsync(&chunks, cs.as<uint64_t>() + qn_, 8, hipMemcpyDeviceToHost,
+                              ctx.stream));
+      ctx.sync();
+      if (chunks) {
+        ms::k_sssp_relax<<<grid_cap(ctx, chunks * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+            out, queue_.as<uint32_t>(), (uint32_t)qn_, cs.as<uint64_t>(), chunks,
+            front_.as<uint64_t>(), next_.as<uint64_t>(), vals_.as<uint32_t>());
+        GRUS_LAUNCH_CHECK();
+      }
+    }
+    ms::k_sssp_finalize<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+        out.off, front_.as<uint64_t>(), next_.as<uint64_t>(), V, running, queue_.as<uint32_t>(),
+        cnt);
+    GRUS_LAUNCH_CHECK();
+  }
+  MsCounters h;
+  GRUS_HIP(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  t_++;
+  qn_ = h.qcount;
+  front_edges_ = h.front_edges;
+  for (int i = 0; i < lanes_; i++) {
+    if (((running >> i) & 1ull) && h.lane_new[i] == 0) {
+      done.push_back(i);
+      finished_mask_ |= 1ull << i;
+    }
+  }
+  return done;
+}
+
+void MsEngine::extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t *checksum,
+                       uint32_t *device_out) {
+  Context &ctx = *ctx_;
+  ctx.activate();
+  const uint64_t V = g_.V;
+  uint32_t *row;
+  if (kind_ == MsKind::BFS) {
+    row = vals_.as<uint32_t>() + (uint64_t)lane * V;
+  } else {
+    row = device_out ? device_out : row_.as<uint32_t>();
+    ms::k_gather_col<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V,
+                                                                       (uint32_t)lane, row);
+    GRUS_LAUNCH_CHECK();
+  }
+  if (device_out && row != device_out)
+    GRUS_HIP(hipMemcpyAsync(device_out, row, V * 4, hipMemcpyDeviceToDevice, ctx.stream));
+  if (reached || checksum) {
+    auto *d = digest_.as<unsigned long long>();
+    GRUS_HIP(hipMemsetAsync(d, 0, 16, ctx.stream));
+    ms::k_digest<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(row, V, d);
+    GRUS_LAUNCH_CHECK();
+    unsigned long long h[2];
+    GRUS_HIP(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, ctx.stream));
+    if (host_out)
+      GRUS_HIP(hipMemcpyAsync(host_out, row, V * 4, hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    if (checksum) *checksum = h[0];
+    if (reached) *reached = h[1];
+  } else if (host_out) {
+    GRUS_HIP(hipMemcpyAsync(host_out, row, V * 4, hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+  }
+}
+
+void MsEngine::release(int lane) {
+  uint64_t bit = 1ull << lane;
+  busy_ &= ~bit;
+  finished_mask_ &= ~bit;
+  dirty_ |= bit;
+}
+
+// ------------------------------------------------------------------ batch API
+namespace {
+void run_multi(Graph &g, MsKind kind, const uint32_t *srcs, int k, uint32_t *out,
+               grus_stats *st) {
+  if (k < 1 || k > 64) raise(GRUS_E_INVALID, "between 1 and 64 sources per batch");
+  if (kind == MsKind::BFS) {
+    if (!g.ms || g.ms->kind() != MsKind::BFS) g.ms = std::make_shared<MsEngine>(g, kind, 64, false);
+  } else {
+    if (!g.ms || g.ms->kind() != MsKind::SSSP) g.ms = std::make_shared<MsEngine>(g, kind, 64, false);
+  }
+  MsEngine &e = *g.ms;
+  Context &ctx = e.ctx();
+  for (int i = 0; i < 64; i++)
+    if ((e.busy_mask() >> i) & 1ull) e.release(i);
+  GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  std::vector<int> lanes = e.admit(srcs, k);
+  uint64_t pending = 0;
+  for (int l : lanes) pending |= 1ull << l;
+  uint64_t steps = 0;
+  while (pending) {
+    if (++steps > g.V + 2) raise(GRUS_E_HIP, "multi-source traversal did not converge");
+    for (int l : e.step()) pending &= ~(1ull << l);
+  }
+  GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  GRUS_HIP(hipEventSynchronize(ctx.ev1));
+  float ms = 0;
+  GRUS_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+  uint64_t vr = 0;
+  for (int i = 0; i < k; i++) {
+    uint64_t r = 0;
+    e.extract(lanes[i], out ? out + (uint64_t)i * g.V : nullptr, &r, nullptr);
+    vr += r;
+  }
+  for (int l : lanes) e.release(l);
+  if (st) {
+    *st = grus_stats{};
+    st->device_ms = ms;
+    st->iterations = (uint32_t)steps;
+    st->vertices_reached = vr;
+  }
+}
+}  // namespace
+
+void run_bfs_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out, grus_stats *st) {
+  run_multi(g, MsKind::BFS, srcs, k, out, st);
+}
+
+void run_sssp_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out, grus_stats *st) {
+  run_multi(g, MsKind::SSSP, srcs, k, out, st);
+}
+
+}  // namespace mgg

@@ -1,0 +1,75 @@
+// multi.hpp -- batched multi-source traversal engines (up to 64 lanes).
+//
+// No reference counterpart: the reference runs one source per process
+// (samples/bfs/bfs.cu:135-173, --src flag).  Here up to 64 queries share one
+// traversal, one bit per query ("lane") in a u64 per vertex (MS-BFS, Then et
+// al., VLDB 2014).  An engine steps all busy lanes together; a lane whose
+// frontier empties is finished, its slot can be reset and given to a new
+// query between steps while the other lanes keep running (continuous
+// batching).  Every lane computes exactly its single-source result:
+//   BFS  level[lane][v] = (step of discovery) + 1 - (step the lane started)
+//   SSSP dist[v][lane]  = Bellman-Ford fixed point of its own source
+// independent of which other lanes share the batch.
+#pragma once
+
+#include <vector>
+
+#include "graph.hpp"
+
+namespace mgg {
+
+enum class MsKind { BFS = 0, SSSP = 1 };
+
+// digest of a device row of V u32 (see grus_result_digest); synchronous
+void row_digest(Context &ctx, const uint32_t *row, uint64_t V, uint64_t *reached,
+                uint64_t *checksum);
+
+struct MsCounters {
+  uint32_t lane_new[64];  // lanes' new frontier sizes this step
+  uint64_t front_edges;   // out-edges of the new frontier (direction heuristic)
+  uint32_t qcount;        // frontier queue length
+  uint32_t long_count;    // pull: vertices handed to wave-cooperative scan
+  uint64_t pad[4];
+};
+
+class MsEngine {
+ public:
+  MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream);
+  ~MsEngine();
+  MsKind kind() const { return kind_; }
+  int lanes() const { return lanes_; }
+  int free_lanes() const;
+  uint64_t busy_mask() const { return busy_; }
+  bool active() const { return busy_ != 0; }
+  // admit sources into free lanes (between steps); returns the lanes used
+  std::vector<int> admit(const uint32_t *srcs, int n);
+  // one expansion step of every busy lane; returns lanes that finished
+  std::vector<int> step();
+  // lane result (V u32) into a device row, then optionally to host memory;
+  // reached = finite entries, checksum = FNV-1a 64 of the V u32 values
+  void extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t *checksum,
+               uint32_t *device_out = nullptr);
+  void release(int lane);  // lane becomes free (reset lazily at next admit)
+  Context &ctx() { return *ctx_; }
+  uint64_t steps() const { return t_; }
+  uint64_t state_bytes() const;
+
+ private:
+  Graph &g_;
+  MsKind kind_;
+  int lanes_;
+  Context *ctx_;
+  std::unique_ptr<Context> own_ctx_;
+  uint64_t busy_ = 0;      // lanes holding a query
+  uint64_t finished_mask_ = 0;  // busy lanes whose frontier has emptied
+  uint64_t dirty_ = 0;     // lanes whose bits must be cleared before reuse
+  uint32_t start_[64] = {};
+  uint64_t t_ = 0;         // global step counter
+  bool pull_ = false;
+  uint64_t qn_ = 0, front_edges_ = 0;
+  Buf visit_, front_, next_, vals_, queue_, longq_, row_, digest_, srcs_dev_, cnt_, push_cnt_;
+  PushWork push_;
+  void reset_lanes(uint64_t mask);
+};
+
+}  // namespace mgg

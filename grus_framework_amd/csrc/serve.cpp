@@ -1,0 +1,209 @@
+// serve.cpp -- continuous-batching query server over the multi-source engines.
+//
+// No reference counterpart (the reference answers one --src per process).
+// The graph stays resident; each query kind (BFS, SSSP) has one or more
+// engines, each owning 1..64 lane slots of device state and its own HIP
+// stream, driven by its own worker thread.  Between two steps of its engine
+// a worker admits waiting queries into free lanes (slot + state memory
+// available), so queries that arrive while others run join the batch in
+// flight, and engines of different kinds overlap on the device.  A lane whose
+// frontier empties is finished: its result is digested (and copied to the
+// caller's buffer when requested) and the slot is freed for the next query.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "multi.hpp"
+#include "serve.hpp"
+
+namespace mgg {
+
+namespace {
+
+double now_us() {
+  using namespace std::chrono;
+  return (double)duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count() /
+         1e3;
+}
+
+struct Query {
+  uint64_t id;
+  int kind;
+  uint32_t src;
+  uint32_t *out;
+  double submit_us, start_us;
+};
+
+class ServerImpl : public Server {
+ public:
+  ServerImpl(Graph &g, const grus_server_opts *o) : g_(g) {
+    opts_.max_inflight_batches = 1;
+    opts_.max_batch = 64;
+    opts_.state_budget_bytes = 0;
+    opts_.copy_results = 0;
+    if (o) {
+      if (o->max_inflight_batches > 0) opts_.max_inflight_batches = o->max_inflight_batches;
+      if (o->max_batch > 0) opts_.max_batch = o->max_batch;
+      opts_.state_budget_bytes = o->state_budget_bytes;
+      opts_.copy_results = o->copy_results;
+    }
+    if (opts_.max_batch > 64) raise(GRUS_E_INVALID, "max_batch is at most 64");
+    if (opts_.max_inflight_batches > 8) raise(GRUS_E_INVALID, "at most 8 engines per kind");
+    // lanes per engine from the state budget: a BFS lane holds a V-entry
+    // level row, an SSSP lane a V-entry distance column
+    int lanes = opts_.max_batch;
+    if (opts_.state_budget_bytes) {
+      uint64_t per_lane = g.V * 4;
+      uint64_t shared = g.V * 8 * 5;
+      int n_eng = 2 * opts_.max_inflight_batches;
+      uint64_t per_engine = opts_.state_budget_bytes / (uint64_t)n_eng;
+      if (per_engine <= shared + per_lane)
+        raise(GRUS_E_BUDGET, "state budget too small for one lane per engine");
+      lanes = (int)std::min<uint64_t>((uint64_t)lanes, (per_engine - shared) / per_lane);
+    }
+    if (!g.symmetric) g.csc();  // build the CSC once, before workers share it
+    for (int k = 0; k < 2; k++)
+      for (int e = 0; e < opts_.max_inflight_batches; e++)
+        engines_.push_back(std::make_unique<MsEngine>(g, (MsKind)k, lanes, true));
+    for (size_t i = 0; i < engines_.size(); i++)
+      workers_.emplace_back([this, i] { this->work(i); });
+  }
+
+  ~ServerImpl() override {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto &t : workers_) t.join();
+  }
+
+  uint64_t submit(int kind, uint32_t src, uint32_t *out) override {
+    if (kind != GRUS_Q_BFS && kind != GRUS_Q_SSSP) raise(GRUS_E_INVALID, "query kind");
+    if (src >= g_.V) raise(GRUS_E_INVALID, "source vertex out of range");
+    std::lock_guard<std::mutex> lk(mu_);
+    if (stop_) raise(GRUS_E_CLOSED, "server is shut down");
+    if (failed_) raise(failed_code_, "server engine failed: " + failed_msg_);
+    Query q{next_id_++, kind, src, out, now_us(), 0.0};
+    pending_[kind].push_back(q);
+    cv_work_.notify_all();
+    return q.id;
+  }
+
+  void poll(grus_query_result *res, int timeout_ms) override {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto ready = [&] { return !done_.empty() || failed_; };
+    if (timeout_ms < 0) {
+      cv_done_.wait(lk, ready);
+    } else if (!cv_done_.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+      raise(GRUS_E_TIMEOUT, "no finished query within the timeout");
+    }
+    if (done_.empty()) raise(failed_code_, "server engine failed: " + failed_msg_);
+    *res = done_.front();
+    done_.pop_front();
+  }
+
+ private:
+  void finish(const Query &q, int status, uint64_t reached, uint64_t checksum) {
+    grus_query_result r{};
+    r.id = q.id;
+    r.status = status;
+    r.reached = reached;
+    r.checksum = checksum;
+    r.submit_us = q.submit_us;
+    r.start_us = q.start_us;
+    r.done_us = now_us();
+    std::lock_guard<std::mutex> lk(mu_);
+    done_.push_back(r);
+    cv_done_.notify_all();
+  }
+
+  void work(size_t ei) {
+    MsEngine &eng = *engines_[ei];
+    const int kind = (int)eng.kind();
+    std::vector<Query> lane_q(64);
+    try {
+      GRUS_HIP(hipSetDevice(g_.ctx->device));
+      for (;;) {
+        std::vector<Query> take;
+        {
+          std::unique_lock<std::mutex> lk(mu_);
+          cv_work_.wait(lk, [&] { return stop_ || !pending_[kind].empty() || eng.active(); });
+          if (stop_ && !eng.active()) break;
+          int room = stop_ ? 0 : eng.free_lanes();
+          while (room-- > 0 && !pending_[kind].empty()) {
+            take.push_back(pending_[kind].front());
+            pending_[kind].pop_front();
+          }
+        }
+        if (!take.empty()) {
+          std::vector<uint32_t> srcs;
+          for (auto &q : take) srcs.push_back(q.src);
+          std::vector<int> lanes = eng.admit(srcs.data(), (int)srcs.size());
+          double t = now_us();
+          for (size_t k = 0; k < lanes.size(); k++) {
+            take[k].start_us = t;
+            lane_q[lanes[k]] = take[k];
+          }
+        }
+        if (!eng.active()) continue;
+        for (int lane : eng.step()) {
+          const Query &q = lane_q[lane];
+          uint64_t reached = 0, checksum = 0;
+          eng.extract(lane, opts_.copy_results ? q.out : nullptr, &reached, &checksum);
+          eng.release(lane);
+          finish(q, GRUS_OK, reached, checksum);
+        }
+      }
+    } catch (const Error &e) {
+      fail(e.code, e.what());
+    } catch (const std::exception &e) {
+      fail(GRUS_E_HIP, e.what());
+    }
+    // queries still held by this engine or waiting for its kind fail
+    std::vector<Query> orphans;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (int i = 0; i < 64; i++)
+        if ((eng.busy_mask() >> i) & 1ull) orphans.push_back(lane_q[i]);
+      if (stop_) {
+        for (auto &q : pending_[kind]) orphans.push_back(q);
+        pending_[kind].clear();
+      }
+    }
+    for (auto &q : orphans) finish(q, stop_ ? GRUS_E_CLOSED : failed_code_, 0, 0);
+  }
+
+  void fail(int code, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(mu_);
+    failed_ = true;
+    failed_code_ = code;
+    failed_msg_ = msg;
+    cv_done_.notify_all();
+  }
+
+  Graph &g_;
+  grus_server_opts opts_{};
+  std::vector<std::unique_ptr<MsEngine>> engines_;
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<Query> pending_[2];
+  std::deque<grus_query_result> done_;
+  bool stop_ = false, failed_ = false;
+  int failed_code_ = GRUS_OK;
+  std::string failed_msg_;
+  uint64_t next_id_ = 1;
+};
+
+}  // namespace
+
+std::unique_ptr<Server> make_server(Graph &g, const grus_server_opts *opts) {
+  g.ctx->activate();
+  return std::make_unique<ServerImpl>(g, opts);
+}
+
+}  // namespace mgg

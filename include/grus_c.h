@@ -177,6 +177,11 @@ int grus_pagerank(grus_graph *g, float alpha, int iters, float *rank_out,
 int grus_cc(grus_graph *g, int method, uint32_t *labels_out, grus_stats *stats);
 /* Copy the last app result held on the device for g (V elements of 4 B). */
 int grus_result_copy(grus_graph *g, void *out);
+/* Digest of the last u32 result held for g (BFS/SSSP/CC): reached = entries
+ * below GRUS_INFINIT, checksum = sum over v of mix64(v << 32 | value)
+ * mod 2^64 (mix64 = the splitmix64 output function); order independent, so
+ * a served query's digest can be compared with a single-query run. */
+int grus_result_digest(grus_graph *g, uint64_t *reached, uint64_t *checksum);
 
 /* ---- batched multi-source (no reference counterpart: the north-star
  * batching pattern; each source gives exactly the single-source result) ----
@@ -204,7 +209,7 @@ typedef struct grus_query_result {
   uint64_t id;
   int status;            /* GRUS_OK or error */
   uint64_t reached;      /* vertices with a finite value */
-  uint64_t checksum;     /* FNV-1a 64 of the V-element result */
+  uint64_t checksum;     /* digest of the V-element result, as grus_result_digest */
   double submit_us, start_us, done_us; /* server clock */
 } grus_query_result;
 int grus_server_create(grus_graph *g, const grus_server_opts *opts, grus_server **out);

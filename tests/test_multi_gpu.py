@@ -1,0 +1,173 @@
+"""GPU parity of the batched multi-source engines and the query server.
+
+Every lane of a batch must equal the single-source oracle result for its
+source, whatever else shares the batch; served queries must give the same
+result alone and in a mix.
+"""
+import numpy as np
+import pytest
+
+import grus_framework_amd as grus
+from grus_framework_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+INF = grus.INFINIT
+
+
+def host_csr(g):
+    from oracle import oracle_py as O
+    off, col, w = g.csr()
+    return O.CSR(off, col, w)
+
+
+def sources(h, k, seed):
+    deg = h.degrees()
+    rng = np.random.default_rng(seed)
+    nz = np.nonzero(deg)[0]
+    s = [int(np.argmax(deg))] + [int(x) for x in rng.choice(nz, k - 3, replace=False)]
+    s.append(int(np.nonzero(deg == 0)[0][0]) if (deg == 0).any() else 0)  # isolated
+    s.append(s[1])  # duplicate source in two lanes
+    return s
+
+
+@pytest.mark.parametrize("k", [1, 7, 64])
+def test_bfs_multi_rmat(ctx, oracle, k):
+    with grus.Graph.rmat(ctx, 14) as g:
+        h = host_csr(g)
+        srcs = sources(h, max(k, 3), seed=k)[:k]
+        out, st = grus.bfs_multi(g, srcs)
+        for i, s in enumerate(srcs):
+            assert (out[i] == oracle.bfs(h, s)).all(), (i, s)
+
+
+def test_bfs_multi_lane_reuse_and_grid(ctx, oracle):
+    R, C = 90, 70
+    with grus.Graph.grid(ctx, R, C) as g:
+        rr, cc = np.divmod(np.arange(R * C), C)
+        srcs = [0, 5 * C + 7, R * C - 1]
+        out, _ = grus.bfs_multi(g, srcs)
+        for i, s in enumerate(srcs):
+            sr, sc = divmod(s, C)
+            assert (out[i] == np.abs(rr - sr) + np.abs(cc - sc)).all()
+        # second batch on the same engine: lanes are reset before reuse
+        out2, _ = grus.bfs_multi(g, [C + 1] * 2 + [17])
+        assert (out2[2] == np.abs(rr - 0) + np.abs(cc - 17)).all()
+
+
+def test_bfs_multi_directed(ctx, oracle):
+    rng = np.random.default_rng(8)
+    V = 4000
+    u = rng.integers(0, V, 25000)
+    v = (u + rng.integers(1, 40, 25000)) % V
+    key = np.unique(u.astype(np.uint64) << np.uint64(32) | v.astype(np.uint64))
+    src = (key >> np.uint64(32)).astype(np.int64)
+    col = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    off = np.zeros(V + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=False) as g:
+        srcs = list(range(0, V, 97))[:40]
+        out, _ = grus.bfs_multi(g, srcs)
+        for i, s in enumerate(srcs):
+            assert (out[i] == oracle.bfs(h, s)).all()
+
+
+@pytest.mark.parametrize("k", [1, 64])
+def test_sssp_multi_rmat(ctx, oracle, k):
+    with grus.Graph.rmat(ctx, 12, weights=(1, 255), wseed=3) as g:
+        h = host_csr(g)
+        srcs = sources(h, max(k, 3), seed=3 + k)[:k]
+        out, _ = grus.sssp_multi(g, srcs)
+        for i, s in enumerate(srcs):
+            assert (out[i] == oracle.sssp(h, s)).all(), (i, s)
+
+
+def test_sssp_multi_grid(ctx, oracle):
+    with grus.Graph.grid(ctx, 60, 80, weights=(1, 1000), wseed=2) as g:
+        h = host_csr(g)
+        srcs = [0, 1234, 4799, 2400]
+        out, _ = grus.sssp_multi(g, srcs)
+        for i, s in enumerate(srcs):
+            assert (out[i] == oracle.sssp(h, s)).all()
+
+
+# ------------------------------------------------------------------ serving
+def single_digests(g, kind, srcs):
+    d = {}
+    for s in set(srcs):
+        (grus.bfs if kind == grus.Q_BFS else grus.sssp)(g, s, copy=False)
+        d[s] = g.digest()
+    return d
+
+
+def test_server_mix_matches_single_queries(ctx, oracle):
+    with grus.Graph.rmat(ctx, 13, weights=(1, 255), wseed=3) as g:
+        h = host_csr(g)
+        rng = np.random.default_rng(11)
+        nz = np.nonzero(h.degrees())[0]
+        qs = [(int(rng.integers(0, 2)), int(rng.choice(nz))) for _ in range(300)]
+        ref = {grus.Q_BFS: single_digests(g, grus.Q_BFS, [s for k, s in qs if k == 0]),
+               grus.Q_SSSP: single_digests(g, grus.Q_SSSP, [s for k, s in qs if k == 1])}
+        outs = {}
+        with grus.Server(g, copy_results=True) as srv:
+            ids = {}
+            for kind, s in qs:
+                buf = np.zeros(g.V, np.uint32)
+                qid = srv.submit(kind, s, buf)
+                ids[qid] = (kind, s)
+                outs[qid] = buf
+            got = {}
+            for _ in range(len(qs)):
+                r = srv.poll(60000)
+                assert r.status == 0
+                got[r.id] = r
+        assert set(got) == set(ids)
+        for qid, (kind, s) in ids.items():
+            r = got[qid]
+            assert (r.reached, r.checksum) == ref[kind][s], (kind, s)
+            assert r.submit_us <= r.start_us <= r.done_us
+        # full results of a sample against the oracle
+        for qid in list(ids)[:12]:
+            kind, s = ids[qid]
+            exp = oracle.bfs(h, s) if kind == grus.Q_BFS else oracle.sssp(h, s)
+            assert (outs[qid] == exp).all()
+
+
+def test_server_alone_equals_in_mix(ctx):
+    with grus.Graph.rmat(ctx, 12, weights=(1, 255), wseed=5) as g:
+        with grus.Server(g) as srv:
+            srv.submit(grus.Q_SSSP, 7)
+            alone = srv.poll(60000)
+            for s in range(100, 160):
+                srv.submit(grus.Q_SSSP, s)
+            srv.submit(grus.Q_SSSP, 7)
+            for s in range(200, 230):
+                srv.submit(grus.Q_BFS, s)
+            res = [srv.poll(60000) for _ in range(91)]
+        mixed = [r for r in res if r.id == 62][0]
+        assert (mixed.reached, mixed.checksum) == (alone.reached, alone.checksum)
+
+
+def test_server_errors(ctx):
+    with grus.Graph.rmat(ctx, 10) as g:
+        with pytest.raises(grus.GrusError) as e:
+            grus.Server(g, state_budget_bytes=1024)
+        assert e.value.code == capi.E_BUDGET
+        with grus.Server(g, max_batch=4) as srv:
+            with pytest.raises(grus.GrusError) as e:
+                srv.submit(grus.Q_BFS, g.V + 5)
+            assert e.value.code == capi.E_INVALID
+            with pytest.raises(grus.GrusError) as e:
+                srv.poll(10)
+            assert e.value.code == capi.E_TIMEOUT
+            for s in range(20):  # more queries than lanes: admitted as lanes free up
+                srv.submit(grus.Q_BFS, s)
+            res = [srv.poll(60000) for _ in range(20)]
+            assert all(r.status == 0 for r in res)
+        # destroy with queries still queued must not hang or crash
+        srv = grus.Server(g, max_batch=2)
+        for s in range(50):
+            srv.submit(grus.Q_SSSP if s % 2 else grus.Q_BFS, s)
+        srv.close()
