@@ -475,6 +475,16 @@ std::unique_ptr<Graph> graph_gen_grid(Context *ctx, uint32_t R, uint32_t C, uint
   return g;
 }
 
+void csr_from_sorted_keys(Context &ctx, const uint64_t *keys, uint64_t m, uint64_t V,
+                          uint64_t *off, uint32_t *col) {
+  k_offsets_from_sorted<<<grid_for(&ctx, m + 1), BLOCK, 0, ctx.stream>>>(keys, m, V, off);
+  GRUS_LAUNCH_CHECK();
+  if (m) {
+    k_low32<<<grid_for(&ctx, m), BLOCK, 0, ctx.stream>>>(keys, m, col);
+    GRUS_LAUNCH_CHECK();
+  }
+}
+
 void graph_copy_csr(Graph &g, uint64_t *off, uint32_t *col, uint32_t *w) {
   g.ctx->activate();
   g.ctx->sync();

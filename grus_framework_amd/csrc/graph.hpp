@@ -145,6 +145,9 @@ std::unique_ptr<Graph> graph_gen_rmat(Context *ctx, int scale, int ef, double a,
 std::unique_ptr<Graph> graph_gen_grid(Context *ctx, uint32_t rows, uint32_t cols, uint32_t wlo,
                                       uint32_t whi, uint64_t wseed, const grus_place *place);
 void graph_copy_csr(Graph &g, uint64_t *off, uint32_t *col, uint32_t *w);
+// CSR (off[V+1], col[m]) from m sorted (row << 32 | col) keys (asynchronous)
+void csr_from_sorted_keys(Context &ctx, const uint64_t *keys, uint64_t m, uint64_t V,
+                          uint64_t *off, uint32_t *col);
 // (re)place a device-built CSR per the placement request
 void graph_apply_placement(Graph &g, const grus_place *place);
 

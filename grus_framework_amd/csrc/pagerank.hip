@@ -29,6 +29,7 @@
 
 #include "advance.hpp"
 #include "apps.hpp"
+#include "prims.hpp"
 
 namespace mgg {
 
@@ -39,6 +40,9 @@ struct PrWork {
   uint64_t nwin = 0;
   const void *built_for = nullptr;  // offsets the window table was built on
   uint64_t cap_v = 0;
+  // degree-sorted relabelled copy (symmetric, HBM-resident graphs)
+  Buf r_off, r_col, perm;  // perm[new id] = original id
+  bool relabelled = false;
 };
 
 namespace pr {
@@ -55,6 +59,41 @@ __global__ void k_window_rows(const uint64_t *off, uint64_t V, uint64_t nwin, ui
   }
 }
 
+// key = (degree descending, id ascending) for the relabelling sort
+__global__ void k_degree_keys(const uint64_t *off, uint64_t V, uint64_t *keys) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V; i += stride) keys[i] = ((0xFFFFFFFFull - (off[i + 1] - off[i])) << 32) | i;
+}
+
+__global__ void k_perm_from_keys(const uint64_t *keys, uint64_t V, uint32_t *perm,
+                                 uint32_t *inv) {
+  uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; r < V; r += stride) {
+    uint32_t v = (uint32_t)keys[r];
+    perm[r] = v;
+    inv[v] = (uint32_t)r;
+  }
+}
+
+// relabelled edge keys (inv[u] << 32 | inv[v]), one wave per row
+__global__ void k_relabel_edges(CsrView g, const uint32_t *inv, uint64_t *keys) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  for (uint64_t u = wave; u < g.V; u += nw) {
+    uint64_t nu = (uint64_t)inv[u] << 32;
+    for (uint64_t e = g.off[u] + lane_id(); e < g.off[u + 1]; e += WAVE)
+      keys[e] = nu | inv[edge_col(g, e)];
+  }
+}
+
+__global__ void k_scatter_back(const float *r_new, const uint32_t *perm, uint64_t V, float *out) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V; i += stride) out[perm[i]] = r_new[i];
+}
+
 __global__ void k_init_contrib(const uint64_t *out_off, uint64_t V, float base, float *contrib) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -66,6 +105,28 @@ __global__ void k_init_contrib(const uint64_t *out_off, uint64_t V, float base, 
 
 __device__ __forceinline__ float finish_row(float sum, float base, float alpha) {
   return base + alpha * sum;
+}
+
+constexpr uint32_t PR_LONG = 16;  // owned-row segments longer than this go to a wave
+
+// a completed row writes its rank (last iteration) or next contrib; a row
+// that continues past the window leaves its partial for the fixup pass
+template <bool kLast>
+__device__ __forceinline__ void store_row(uint32_t r, float s, bool partial, uint64_t k,
+                                          const CsrView &in, const uint64_t *out_off,
+                                          float *next, float *tail_sum, float base,
+                                          float alpha) {
+  if (partial) {
+    tail_sum[k] = s;
+    return;
+  }
+  float rank = finish_row(s, base, alpha);
+  if (kLast) {
+    next[r] = rank;
+  } else {
+    uint64_t d = out_off[r + 1] - out_off[r];
+    next[r] = d ? rank / (float)d : 0.0f;
+  }
 }
 
 template <bool kLast>
@@ -86,7 +147,12 @@ __global__ __launch_bounds__(BLOCK) void k_pull_window(CsrView in, const uint64_
   for (int j = 0; j < PR_WIN / BLOCK; j++) {
     uint64_t e = e0 + (uint64_t)j * BLOCK + threadIdx.x;
     float x = 0.0f;
-    if (e < e1) x = contrib[edge_col(in, e)];
+    if (e < e1) {
+      // the column stream is read once per iteration: load it non-temporally
+      // so it does not evict the contrib lines that later windows reuse
+      uint32_t c = e < in.split ? __builtin_nontemporal_load(&in.col[e]) : in.col_far[e - in.split];
+      x = contrib[c];
+    }
     vals[j * BLOCK + threadIdx.x] = x;
   }
   __syncthreads();
@@ -106,23 +172,35 @@ __global__ __launch_bounds__(BLOCK) void k_pull_window(CsrView in, const uint64_
       head_sum[k] = t;
     }
   }
-  // owned rows
+  // owned rows: short segments are summed by one lane, long ones (queued in
+  // LDS) by a whole wave with a fixed shuffle tree, so a window dominated by
+  // a few long rows does not serialise on one lane
+  __shared__ uint32_t s_long[PR_WIN / PR_LONG + 1];
+  __shared__ uint32_t s_nlong;
+  if (threadIdx.x == 0) s_nlong = 0;
+  __syncthreads();
   for (uint32_t r = r_begin + threadIdx.x; r < r_end; r += BLOCK) {
     uint64_t a = in.off[r], b = in.off[r + 1];
     uint64_t bb = b < e1 ? b : e1;
+    if (bb - a > PR_LONG) {
+      s_long[atomicAdd(&s_nlong, 1u)] = r;
+      continue;
+    }
     float s = 0.0f;
     for (uint64_t e = a; e < bb; e++) s += vals[e - e0];
-    if (b > e1) {
-      tail_sum[k] = s;  // completed by the fixup pass
-    } else {
-      float rank = finish_row(s, base, alpha);
-      if (kLast) {
-        next[r] = rank;
-      } else {
-        uint64_t d = out_off[r + 1] - out_off[r];
-        next[r] = d ? rank / (float)d : 0.0f;
-      }
-    }
+    store_row<kLast>(r, s, b > e1, k, in, out_off, next, tail_sum, base, alpha);
+  }
+  __syncthreads();
+  const uint32_t nlong = s_nlong;
+  for (uint32_t i = threadIdx.x / WAVE; i < nlong; i += BLOCK / WAVE) {
+    const uint32_t r = s_long[i];
+    uint64_t a = in.off[r], b = in.off[r + 1];
+    uint64_t bb = b < e1 ? b : e1;
+    float s = 0.0f;
+    for (uint64_t e = a + lane_id(); e < bb; e += WAVE) s += vals[e - e0];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane_id() == 0) store_row<kLast>(r, s, b > e1, k, in, out_off, next, tail_sum, base, alpha);
   }
 }
 
@@ -154,6 +232,38 @@ __global__ void k_fixup(CsrView in, const uint64_t *out_off, const uint32_t *win
 
 }  // namespace pr
 
+namespace {
+// Degree-sorted relabelling (one-time, cached): new id = rank of the vertex
+// by degree (descending, ties by id), so the contributions of high-degree
+// vertices -- the targets of most gathers on a power-law graph -- share cache
+// lines instead of being spread over the whole contrib array.
+void build_relabelled(Graph &g, PrWork &w) {
+  Context &ctx = *g.ctx;
+  const uint64_t V = g.V, E = g.E;
+  Buf ka, kb, inv;
+  ka.alloc(std::max(V, E) * 8);
+  kb.alloc(std::max(V, E) * 8);
+  inv.alloc(V * 4);
+  w.perm.alloc(V * 4);
+  pr::k_degree_keys<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(g.off.as<uint64_t>(), V,
+                                                                      ka.as<uint64_t>());
+  GRUS_LAUNCH_CHECK();
+  uint64_t *sk = sort_keys_u64(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), V);
+  pr::k_perm_from_keys<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+      sk, V, w.perm.as<uint32_t>(), inv.as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  pr::k_relabel_edges<<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
+      g.csr(), inv.as<uint32_t>(), ka.as<uint64_t>());
+  GRUS_LAUNCH_CHECK();
+  uint64_t *se = sort_keys_u64(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), E);
+  w.r_off.alloc((V + 1) * 8);
+  w.r_col.alloc(E * 4);
+  csr_from_sorted_keys(ctx, se, E, V, w.r_off.as<uint64_t>(), w.r_col.as<uint32_t>());
+  ctx.sync();
+  w.relabelled = true;
+}
+}  // namespace
+
 void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
   Context &ctx = *g.ctx;
   ctx.activate();
@@ -163,8 +273,21 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
   if (!g.pr) g.pr = std::make_shared<PrWork>();
   PrWork &w = *g.pr;
   const uint64_t V = g.V;
-  const CsrView out = g.csr();
-  const CsrView in = g.csc();
+  const uint32_t mode = ctx.tuning.pr_relabel;
+  // off by default: on RMAT inputs the natural numbering already spreads the
+  // hubs over many cache lines, while degree order packs the hottest
+  // contributions into a few lines and measured ~5x slower (L2 hot-spotting)
+  const bool relabel = g.symmetric && g.placement == GRUS_PLACE_HBM && mode == 1;
+  if (relabel && !w.relabelled) build_relabelled(g, w);
+  CsrView out = g.csr();
+  CsrView in = g.csc();
+  if (relabel) {
+    out.off = w.r_off.as<const uint64_t>();
+    out.col = w.r_col.as<const uint32_t>();
+    out.w = nullptr;
+    out.split = out.E;
+    in = out;
+  }
   if (w.built_for != in.off || w.cap_v < V) {
     w.nwin = in.E / PR_WIN + 1;
     w.win_row.alloc((w.nwin + 1) * 4);
@@ -197,7 +320,7 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
   int cur = 0;
   for (int it = 0; it < iters; it++) {
     const bool last = (it == iters - 1);
-    float *next = last ? rank : w.contrib[cur ^ 1].as<float>();
+    float *next = (last && !relabel) ? rank : w.contrib[cur ^ 1].as<float>();
     const float *c = w.contrib[cur].as<float>();
     if (last) {
       pr::k_pull_window<true><<<nwin, BLOCK, 0, ctx.stream>>>(
@@ -219,6 +342,12 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
     GRUS_LAUNCH_CHECK();
     launches += 2;
     cur ^= 1;
+  }
+  if (relabel && iters > 0) {  // ranks back to the original vertex ids
+    pr::k_scatter_back<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+        w.contrib[cur].as<float>(), w.perm.as<uint32_t>(), V, rank);
+    GRUS_LAUNCH_CHECK();
+    launches++;
   }
   GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   GRUS_HIP(hipEventSynchronize(ctx.ev1));

@@ -92,7 +92,9 @@ typedef struct grus_tuning {
   uint32_t bu_thread_edges; /* bottom-up: edges a lane scans before handing
                                the vertex to a wave (default 32) */
   uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic) */
-  uint32_t reserved[8];
+  uint32_t pr_relabel;    /* PageRank on a degree-sorted relabelled copy of a
+                             symmetric HBM graph: 1 on; 0 (default) or 2 off */
+  uint32_t reserved[7];
 } grus_tuning;
 
 /* ---- errors / version ---- */

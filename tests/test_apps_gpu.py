@@ -206,3 +206,17 @@ def test_apps_rmat_s20_vs_oracle(ctx, oracle):
         print("s20 pagerank", flush=True)
         r, _ = grus.pagerank(g)
         assert np.abs(r - oracle.pagerank(h)).sum() <= PR_TOL
+
+
+def test_pagerank_relabelled_path(ctx, oracle):
+    with grus.Graph.rmat(ctx, 14) as g:
+        h = host_csr(g)
+        t0 = ctx.tuning
+        try:
+            t = ctx.tuning
+            t.pr_relabel = 1
+            ctx.tuning = t
+            r, _ = grus.pagerank(g)
+        finally:
+            ctx.tuning = t0
+        assert np.abs(r - oracle.pagerank(h)).sum() <= PR_TOL

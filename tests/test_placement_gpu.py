@@ -1,0 +1,66 @@
+"""GPU parity across placements: HBM, managed (with the reference's hints) and
+capped (edge prefix in HBM, remainder in pinned host memory) must give the
+same BFS levels, SSSP distances, CC labels and PageRank ranks."""
+import numpy as np
+import pytest
+
+import grus_framework_amd as grus
+from grus_framework_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def run_all(g, src):
+    return (grus.bfs(g, src)[0], grus.bfs(g, src, "push")[0], grus.sssp(g, src)[0],
+            grus.cc(g)[0], grus.pagerank(g)[0])
+
+
+@pytest.fixture(scope="module")
+def reference(ctx):
+    g = grus.Graph.rmat(ctx, 15, weights=(1, 255), wseed=3)
+    src = g.info.max_degree_vertex
+    out = run_all(g, src)
+    csr = g.csr()
+    g.close()
+    return src, csr, out
+
+
+def check_same(got, ref):
+    for a, b in zip(got[:4], ref[:4]):
+        assert (a == b).all()
+    assert np.abs(got[4] - ref[4]).sum() <= 1e-6
+
+
+@pytest.mark.parametrize("frac", [0.0, 0.3, 0.9])
+def test_capped_matches_hbm(ctx, reference, frac):
+    src, (off, col, w), ref = reference
+    V, E = len(off) - 1, len(col)
+    cap = (V + 1) * 8 + int(frac * E) * 8  # offsets + frac of (col, weight) pairs
+    pl = grus.place("capped", hbm_cap_bytes=cap)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=True, placement=pl) as g:
+        assert g.info.placement == grus.PLACE_CAPPED
+        assert g.info.host_bytes == (E - int(frac * E)) * 8
+        check_same(run_all(g, src), ref)
+        o2, c2, w2 = g.csr()
+        assert (c2 == col).all() and (w2 == w).all()
+
+
+@pytest.mark.parametrize("flags", [dict(), dict(pf=True), dict(ab=True, rm=True),
+                                   dict(pf=True, pl=True, ab=True, rm=True)])
+def test_managed_matches_hbm(ctx, reference, flags):
+    src, (off, col, w), ref = reference
+    pl = grus.place("managed", **flags)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=True, placement=pl) as g:
+        assert g.info.placement == grus.PLACE_MANAGED
+        check_same(run_all(g, src), ref)
+
+
+def test_capped_rmat_generator_and_errors(ctx, reference):
+    src, _, ref = reference
+    pl = grus.place("capped", hbm_cap_bytes=4 << 20)
+    with grus.Graph.rmat(ctx, 15, weights=(1, 255), wseed=3, placement=pl) as g:
+        assert g.info.host_bytes > 0
+        check_same(run_all(g, src), ref)
+    with pytest.raises(grus.GrusError) as e:
+        grus.Graph.rmat(ctx, 15, placement=grus.place("capped", hbm_cap_bytes=1024))
+    assert e.value.code == capi.E_BUDGET
