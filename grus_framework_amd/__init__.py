@@ -15,7 +15,7 @@ from . import capi
 from .capi import (DIR_AUTO, DIR_PULL, DIR_PUSH, INFINIT, PLACE_CAPPED, PLACE_HBM,
                    PLACE_MANAGED, Q_BFS, Q_SSSP, GrusError, Place, Stats, Tuning)
 
-__all__ = ["Context", "Graph", "Server", "bfs", "sssp", "pagerank", "cc", "bfs_multi",
+__all__ = ["Context", "Graph", "Server", "autotune", "bfs", "sssp", "pagerank", "cc", "bfs_multi",
            "sssp_multi", "gr_header", "gr_read", "gr_write", "device_count", "GrusError",
            "INFINIT", "Place", "Stats", "Tuning", "place"]
 
@@ -225,6 +225,36 @@ def sssp_multi(g, srcs, copy=True):
     out = np.zeros((len(srcs), g.V), np.uint32) if copy else None
     capi.check(_lib().grus_sssp_multi(g.h, _p(srcs), len(srcs), _p(out), C.byref(st)))
     return out, st
+
+
+def autotune(g, kind="bfs", srcs=(), cache=None):
+    """Pick the fastest thresholds for g (grus_autotune) and install them in
+    g's context.  With `cache` (a JSON file path) the result is stored under
+    the graph signature and reused on later calls."""
+    import json
+    import os
+    kinds = {"bfs": capi.TUNE_BFS, "sssp": capi.TUNE_SSSP, "pr": capi.TUNE_PR}
+    sig = C.c_uint64()
+    capi.check(_lib().grus_graph_signature(g.h, C.byref(sig)))
+    key = "%s:%016x" % (kind, sig.value)
+    store = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            store = json.load(f)
+    t = g.ctx.tuning
+    if key in store:
+        for name, val in store[key].items():
+            setattr(t, name, val)
+        g.ctx.tuning = t
+        return t
+    srcs = np.ascontiguousarray(srcs, np.uint32)
+    capi.check(_lib().grus_autotune(g.h, kinds[kind], _p(srcs) if len(srcs) else None, len(srcs),
+                                    C.byref(t)))
+    if cache:
+        store[key] = {n: getattr(t, n) for n, _ in Tuning._fields_ if n != "reserved"}
+        with open(cache, "w") as f:
+            json.dump(store, f, indent=1)
+    return t
 
 
 class Server:

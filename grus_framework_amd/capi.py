@@ -18,6 +18,7 @@ E_BUDGET, E_TIMEOUT, E_UNSUPPORTED, E_NODEVICE, E_CLOSED = -6, -7, -8, -9, -10
 PLACE_HBM, PLACE_MANAGED, PLACE_CAPPED = 0, 1, 2
 DIR_AUTO, DIR_PUSH, DIR_PULL = 0, 1, 2
 Q_BFS, Q_SSSP = 0, 1
+TUNE_BFS, TUNE_SSSP, TUNE_PR = 0, 1, 2
 
 
 class Place(C.Structure):
@@ -93,6 +94,8 @@ SIGNATURES = [
     ("grus_result_digest", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
     ("grus_bfs_multi", _i, [_vp, _vp, _i, _vp, C.POINTER(Stats)]),
     ("grus_sssp_multi", _i, [_vp, _vp, _i, _vp, C.POINTER(Stats)]),
+    ("grus_autotune", _i, [_vp, _i, _vp, _i, C.POINTER(Tuning)]),
+    ("grus_graph_signature", _i, [_vp, C.POINTER(_u64)]),
     ("grus_server_create", _i, [_vp, C.POINTER(ServerOpts), _pp]),
     ("grus_server_submit", _i, [_vp, _i, _u32, _vp, C.POINTER(_u64)]),
     ("grus_server_poll", _i, [_vp, C.POINTER(QueryResult), _i]),

@@ -17,4 +17,8 @@ void run_cc(Graph &g, int method, grus_stats *st);
 void run_bfs_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);
 void run_sssp_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);
 
+// autotuner (tune.cpp)
+void autotune(Graph &g, int kind, const uint32_t *srcs, int n, grus_tuning *out);
+uint64_t graph_signature(const Graph &g);
+
 }  // namespace mgg

@@ -302,6 +302,23 @@ int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
   });
 }
 
+// ---------------------------------------------------------------- tuning
+int grus_autotune(grus_graph *g, int kind, const uint32_t *srcs, int n, grus_tuning *out) {
+  return guard([&] {
+    need(g, "g");
+    mgg::autotune(*g->g, kind, srcs, n, out);
+    g->last_result = 0;
+  });
+}
+
+int grus_graph_signature(grus_graph *g, uint64_t *sig) {
+  return guard([&] {
+    need(g, "g");
+    need(sig, "sig");
+    *sig = mgg::graph_signature(*g->g);
+  });
+}
+
 // ---------------------------------------------------------------- serving
 int grus_server_create(grus_graph *g, const grus_server_opts *opts, grus_server **out) {
   return guard([&] {

@@ -97,6 +97,19 @@ typedef struct grus_tuning {
   uint32_t reserved[7];
 } grus_tuning;
 
+/* ---- autotuning (no reference counterpart: the reference hard-codes its
+ * thresholds, include/frontier.cuh:140) ----
+ * Times the app on g under candidate settings (BFS: direction-switch alpha,
+ * beta and the bottom-up lane/wave cut-off; SSSP: bucket width; PR:
+ * relabelling) for the sample sources, keeps the fastest in the context's
+ * tuning and returns it.  Results never depend on the tuning. */
+#define GRUS_TUNE_BFS 0
+#define GRUS_TUNE_SSSP 1
+#define GRUS_TUNE_PR 2
+int grus_autotune(grus_graph *g, int kind, const uint32_t *srcs, int n, grus_tuning *out);
+/* Key for caching a tuning per graph (V, E, degree extremes, flags). */
+int grus_graph_signature(grus_graph *g, uint64_t *sig);
+
 /* ---- errors / version ---- */
 const char *grus_last_error(void);
 const char *grus_version(void);

@@ -1,0 +1,72 @@
+"""Command-line tools and the autotuner on the GPU."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import grus_framework_amd as grus
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "grus_framework_amd", "bin")
+
+
+def load_dump(path, dtype):
+    vals = np.loadtxt(path, dtype=np.float64)
+    assert (vals[:, 0] == np.arange(len(vals))).all()
+    return vals[:, 1].astype(dtype)
+
+
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pr"])
+def test_cli_dump_matches_oracle(tmp_path, oracle, app):
+    # a .gr input written by the product writer, read back by the CLI
+    g = oracle.gen_rmat(11, weights=(1, 255), wseed=3)
+    gr = str(tmp_path / "g.gr")
+    grus.gr_write(gr, g.off, g.col, g.w)
+    out = str(tmp_path / "out.txt")
+    src = int(np.argmax(g.degrees()))
+    r = subprocess.run([os.path.join(BIN, "grus_" + app), "--input", gr, "--symmetric",
+                        "--src", str(src), "--output", out], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "itr " in r.stdout and "Done saving." in r.stdout
+    if app == "pr":
+        assert np.abs(load_dump(out, np.float64) - oracle.pagerank(g)).sum() <= 1e-6
+    else:
+        ref = {"bfs": lambda: oracle.bfs(g, src), "sssp": lambda: oracle.sssp(g, src),
+               "cc": lambda: oracle.cc(g)}[app]()
+        assert (load_dump(out, np.uint64).astype(np.uint32) == ref).all()
+
+
+def test_cli_placement_flags(tmp_path, oracle):
+    out = str(tmp_path / "o.txt")
+    for extra in (["--noopt", "--pf", "--ab"], ["--placement", "capped", "--hbm_cap_gb", "0.001"]):
+        r = subprocess.run([os.path.join(BIN, "grus_bfs"), "--rmat", "12", "--src", "0",
+                            "--output", out] + extra, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        g = oracle.gen_rmat(12)
+        assert (load_dump(out, np.uint64).astype(np.uint32) == oracle.bfs(g, 0)).all()
+
+
+def test_autotune_keeps_results_and_caches(ctx, tmp_path):
+    cache = str(tmp_path / "tune.json")
+    with grus.Graph.rmat(ctx, 16, weights=(1, 255), wseed=3) as g:
+        s = [g.info.max_degree_vertex, 3, 77]
+        before = [grus.bfs(g, x)[0] for x in s] + [grus.sssp(g, x)[0] for x in s]
+        t0 = ctx.tuning
+        try:
+            tb = grus.autotune(g, "bfs", s, cache=cache)
+            ts = grus.autotune(g, "sssp", s, cache=cache)
+            assert tb.bfs_alpha in (2, 4, 8, 14, 24, 48, 96, 192)
+            after = [grus.bfs(g, x)[0] for x in s] + [grus.sssp(g, x)[0] for x in s]
+            for a, b in zip(before, after):
+                assert (a == b).all()
+            stored = json.load(open(cache))
+            assert len(stored) == 2
+            again = grus.autotune(g, "bfs", s, cache=cache)  # from the cache
+            assert again.bfs_alpha == tb.bfs_alpha and again.sssp_delta == ts.sssp_delta
+        finally:
+            ctx.tuning = t0
