@@ -1,17 +1,22 @@
-"""Benchmark: BFS GTEPS on RMAT scale 24 (one MI355X), plus SSSP / PageRank / CC.
+"""Benchmark of the traversal engine on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extra] [--no-cpu]
 
-Workload (one "step" = one BFS traversal from one source): RMAT scale 24,
-edge factor 16, a/b/c = .57/.19/.19, seed 1, symmetrized, deduplicated,
+Headline workload (one step = one BFS traversal from one source): RMAT scale
+24, edge factor 16, a/b/c = .57/.19/.19, seed 1, symmetrized, deduplicated,
 self-loops dropped (V = 16,777,216, E = 520,761,196 directed edges), built
-and resident in HBM before timing.  Sources: the max-degree vertex and 16
-seeded random non-isolated vertices, cycled.  GTEPS = sum of E_r (directed
-edges of the reached component) over the timed steps / wall time.
+on the device and resident in HBM before timing.  Sources: the max-degree
+vertex and 16 seeded random non-isolated vertices, cycled; the direction
+thresholds are autotuned first on a different seeded sample.  value =
+GTEPS = sum over the timed steps of E_r (directed edges of the reached
+component) / wall time of the timed loop (each call returns when the device
+is idle).
 
-Prints one JSON line.  The roofline object covers the BFS traversal; the CPU
-baseline is the oracle's OpenMP direction-optimizing BFS on this host's
-cores over a bounded sample of the same sources.
+Also reported (the rest of the metric): SSSP GTEPS on the 4096x4096 grid
+and on weighted RMAT s24, PageRank pull GB/s and CC on RMAT s24, batched
+serving on RMAT s22 (queries/s, p50/p99, vs a one-query-at-a-time loop) and
+capped-HBM placement on RMAT s27 -- each next to an OpenMP CPU baseline or
+the resident run where one applies.  One JSON line on stdout.
 """
 import argparse
 import json
@@ -25,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(*a):
@@ -47,6 +53,16 @@ def cpu_model():
     return "unknown"
 
 
+def traffic(key):
+    """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled)."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)[key]
+        return int(t["fetch_bytes"] + t["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,7 +71,10 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the CPU baseline sample")
-    ap.add_argument("--no-extra", action="store_true", help="skip SSSP / PageRank / CC")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only (no SSSP / PageRank / CC / serving / capped)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default thresholds")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -74,6 +93,13 @@ def main():
     h = O.CSR(off, col)
     deg = h.degrees()
     srcs = pick_sources(deg, 16, seed=5)
+    tuned = None
+    if not args.no_tune:
+        tune_srcs = pick_sources(deg, 4, seed=99)[1:]
+        t = grus.autotune(g, "bfs", tune_srcs)
+        tuned = {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta,
+                 "bu_thread_edges": t.bu_thread_edges}
+        log("autotuned on other sources: %s" % tuned)
 
     for i in range(args.warmup):
         grus.bfs(g, srcs[i % len(srcs)], copy=False)
@@ -87,24 +113,28 @@ def main():
         dev_ms += st.device_ms
     wall = time.perf_counter() - t0
     gteps = er / wall / 1e9
-    algo_bytes = 4 * er + 16 * vr + 4 * g.V * args.steps
-    achieved = algo_bytes / (dev_ms * 1e-3) / 1e9
+    algo_bytes = (4 * er + 16 * vr + 4 * g.V * args.steps) / args.steps
+    achieved = algo_bytes / (dev_ms / args.steps * 1e-3) / 1e9
     log("bfs: %d steps %.3f ms/step, %.1f GTEPS (device %.3f ms/step)" %
         (args.steps, wall / args.steps * 1e3, gteps, dev_ms / args.steps))
 
-    # CPU baseline: OpenMP direction-optimizing BFS on a bounded sample
     threads = O.max_threads()
-    cpu_er, cpu_ms, n_cpu = 0, 0.0, 0
-    for s in srcs:
-        lv, ms = O.bl_bfs(h, s, threads)
-        cpu_er += int(deg[lv != O.INFINIT].sum())
-        cpu_ms += ms
-        n_cpu += 1
-        if cpu_ms > args.cpu_seconds * 1e3:
-            break
-    cpu_gteps = cpu_er / (cpu_ms * 1e-3) / 1e9
-    log("cpu bfs: %d sources %.1f ms each, %.2f GTEPS on %d threads (%s)" %
-        (n_cpu, cpu_ms / n_cpu, cpu_gteps, threads, cpu_model()))
+    cpu = None
+    if not args.no_cpu:
+        cpu_er, cpu_ms, n_cpu = 0, 0.0, 0
+        for s in srcs:
+            lv, ms = O.bl_bfs(h, s, threads)
+            cpu_er += int(deg[lv != O.INFINIT].sum())
+            cpu_ms += ms
+            n_cpu += 1
+            if cpu_ms > args.cpu_seconds * 1e3:
+                break
+        cpu_gteps = cpu_er / (cpu_ms * 1e-3) / 1e9
+        log("cpu bfs: %d sources %.1f ms each, %.2f GTEPS on %d threads (%s)" %
+            (n_cpu, cpu_ms / n_cpu, cpu_gteps, threads, cpu_model()))
+        cpu = {"value": round(cpu_gteps, 3), "unit": "GTEPS", "cores": threads, "kind": "port",
+               "sample": "OpenMP direction-optimizing BFS (oracle/baseline_omp.c) from %d of "
+                         "the %d sources on the full graph, %s" % (n_cpu, len(srcs), cpu_model())}
 
     extra = {}
     if not args.no_extra:
@@ -112,7 +142,8 @@ def main():
     g.close()
 
     line = {
-        "metric": "BFS GTEPS at RMAT s24 (directed edges of the reached component / s)",
+        "metric": "BFS GTEPS at RMAT s24 (directed edges of the reached component / s); "
+                  "SSSP GTEPS, PageRank GB/s, serving queries/s in extra",
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,
@@ -124,27 +155,29 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (RMAT generated on the device, seed 1)",
-        "config": {"workload": "bfs_rmat_s%d_ef16" % args.scale, "V": int(g.V),
-                   "E": int(h.E), "sources": len(srcs), "direction": "auto",
-                   "graph500_gteps": round(gteps / 2, 3)},
-        "roofline": {"kernel": "bfs traversal (all launches of one source)", "bound": "hbm",
+        "config": {"workload": "bfs_rmat_s%d_ef16" % args.scale, "V": int(g.V), "E": int(h.E),
+                   "sources": len(srcs), "direction": "auto (autotuned)" if tuned else "auto",
+                   "tuning": tuned, "graph500_gteps": round(gteps / 2, 3)},
+        "roofline": {"kernel": "BFS traversal of one source (all its launches; k_pull "
+                               "dominates)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_step": int(algo_bytes // args.steps)},
-        "cpu_baseline": {"value": round(cpu_gteps, 3), "unit": "GTEPS", "cores": threads,
-                         "kind": "port",
-                         "sample": "OpenMP direction-optimizing BFS from %d of the %d sources "
-                                   "on the full graph (%s)" % (n_cpu, len(srcs), cpu_model())},
-        "gpu_vs_cpu": round(gteps / cpu_gteps, 2) if cpu_gteps else None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic("bfs_rmat_s24_traversal"),
+                     "algorithmic_bytes": int(algo_bytes),
+                     "device_ms": round(dev_ms / args.steps, 4)},
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "extra": extra,
     }
+    if "pagerank" in extra:
+        line["roofline_pagerank"] = extra["pagerank"].pop("roofline")
     print(json.dumps(line), flush=True)
     ctx.close()
 
 
 def run_extra(grus, O, ctx, g, h, threads, args):
     out = {}
-    # PageRank pull, 20 iterations, FP32 (same RMAT graph)
+    # ---- PageRank pull, 20 iterations, FP32 (same RMAT graph) ----
     grus.pagerank(g, copy=False)
     reps = 3
     ms = 0.0
@@ -152,31 +185,148 @@ def run_extra(grus, O, ctx, g, h, threads, args):
         _, st = grus.pagerank(g, copy=False)
         ms += st.device_ms
     ms /= reps
-    B = 20 * (8 * (g.V + 1) + 8 * g.E + 12 * g.V)
-    _, cms = O.bl_pagerank(h, 0.85, 2, threads)  # 2-iteration CPU sample, scaled
-    cpu_gbs = (B / 10) / (cms * 1e-3) / 1e9
-    out["pagerank"] = {"ms_20_iters": round(ms, 3), "GBps_algorithmic": round(B / ms / 1e6, 1),
-                       "frac_hbm_peak": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
-                       "cpu_GBps": round(cpu_gbs, 2), "cpu_sample": "2 iterations, scaled"}
+    per_iter = 8 * (g.V + 1) + 8 * g.E + 12 * g.V
+    B = 20 * per_iter
+    pr = {"ms_20_iters": round(ms, 3), "GBps_algorithmic": round(B / ms / 1e6, 1),
+          "frac_hbm_peak": round(B / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    pr["roofline"] = {"kernel": "pr::k_pull_window (one iteration per launch)", "bound": "hbm",
+                      "achieved": round(per_iter / (ms / 20 * 1e-3) / 1e9, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(per_iter / (ms / 20 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": traffic("pagerank_rmat_s24_iteration"),
+                      "algorithmic_bytes": int(per_iter),
+                      "note": "launch time includes the small fixup kernel"}
+    if not args.no_cpu:
+        _, cms = O.bl_pagerank(h, 0.85, 2, threads)  # 2-iteration CPU sample, scaled
+        pr["cpu_GBps"] = round((B / 10) / (cms * 1e-3) / 1e9, 2)
+        pr["cpu_sample"] = "2 of 20 iterations, %d threads" % threads
+    out["pagerank"] = pr
     log("pagerank: %.3f ms (%.1f GB/s)" % (ms, B / ms / 1e6))
-    # CC (hooking) on the same graph
+
+    # ---- CC (hooking) on the same graph ----
     grus.cc(g, copy=False)
     ms = min(grus.cc(g, copy=False)[1].device_ms for _ in range(3))
-    _, cms = O.bl_cc(h, threads)
-    out["cc"] = {"ms": round(ms, 3), "GTEPS": round(g.E / ms / 1e6, 2), "cpu_ms": round(cms, 1)}
-    log("cc: %.3f ms (cpu %.1f ms)" % (ms, cms))
-    # SSSP on the 4096x4096 weighted grid (source 0)
+    cc = {"ms": round(ms, 3), "GTEPS": round(g.E / ms / 1e6, 2)}
+    if not args.no_cpu:
+        _, cms = O.bl_cc(h, threads)
+        cc["cpu_ms"] = round(cms, 1)
+    out["cc"] = cc
+    log("cc: %.3f ms" % ms)
+
+    # ---- SSSP on weighted RMAT s24 (w in [1,255]) ----
+    with grus.Graph.rmat(ctx, args.scale, weights=(1, 255), wseed=3) as gw:
+        s = gw.info.max_degree_vertex
+        grus.sssp(gw, s, copy=False)
+        _, st = grus.sssp(gw, s, copy=False)
+        out["sssp_rmat_s%d" % args.scale] = {
+            "ms": round(st.device_ms, 2), "GTEPS": round(st.edges_reached / st.device_ms / 1e6, 3),
+            "rounds": st.iterations}
+        log("sssp rmat: %.2f ms" % st.device_ms)
+
+    # ---- SSSP on the 4096x4096 weighted grid (source 0) ----
     with grus.Graph.grid(ctx, 4096, 4096, weights=(1, 1000), wseed=2) as gg:
+        grus.sssp(gg, 0, copy=False)
         _, st = grus.sssp(gg, 0, copy=False)
         ms = st.device_ms
-        gteps = st.edges_reached / ms / 1e6
-        hh = O.CSR(*gg.csr())
-        _, cms = O.bl_sssp(hh, 0, 4000, threads)
-        cpu_gteps = st.edges_reached / cms / 1e6
-    out["sssp_grid4096"] = {"ms": round(ms, 2), "GTEPS": round(gteps, 3), "rounds": st.iterations,
-                            "cpu_ms": round(cms, 1), "cpu_GTEPS": round(cpu_gteps, 3)}
-    log("sssp grid: %.2f ms (cpu %.1f ms)" % (ms, cms))
+        grid = {"ms": round(ms, 2), "GTEPS": round(st.edges_reached / ms / 1e6, 3),
+                "rounds": st.iterations}
+        if not args.no_cpu:
+            hh = O.CSR(*gg.csr())
+            _, cms = O.bl_sssp(hh, 0, 4000, threads)
+            grid["cpu_ms"] = round(cms, 1)
+            grid["cpu_GTEPS"] = round(st.edges_reached / cms / 1e6, 3)
+        out["sssp_grid4096"] = grid
+        log("sssp grid: %.2f ms" % ms)
+
+    out["serving_rmat_s22"] = run_serving(grus)
+    try:
+        out["capped_rmat_s27"] = run_capped(grus, ctx)
+    except grus.GrusError as e:  # e.g. not enough device or host memory
+        out["capped_rmat_s27"] = {"error": str(e)}
     return out
+
+
+def run_serving(grus):
+    """Batched serving on RMAT s22 (w in [1,255]) vs a one-query-at-a-time loop."""
+    ctx = grus.Context(0)
+    g = grus.Graph.rmat(ctx, 22, weights=(1, 255), wseed=3)
+    off, _, _ = g.csr()
+    nz = np.nonzero(np.diff(off.astype(np.int64)))[0]
+    rng = np.random.default_rng(5)
+    N = 2048
+    kinds = rng.integers(0, 2, N)
+    srcs = rng.choice(nz, N)
+    # serial: best single-query kernels, one at a time, digest per query
+    ns = 128
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(ns):
+        q0 = time.perf_counter()
+        (grus.bfs if kinds[i] == 0 else grus.sssp)(g, int(srcs[i]), copy=False)
+        g.digest()
+        lat.append(time.perf_counter() - q0)
+    serial_qps = ns / (time.perf_counter() - t0)
+    res = {"serial_qps": round(serial_qps, 1),
+           "serial_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+           "serial_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2)}
+    # batched at saturation: all queries offered at once
+    with grus.Server(g, max_batch=64, max_inflight_batches=2) as srv:
+        t0 = time.perf_counter()
+        for i in range(N):
+            srv.submit(int(kinds[i]), int(srcs[i]))
+        out = [srv.poll(300000) for _ in range(N)]
+        tb = time.perf_counter() - t0
+    assert all(r.status == 0 for r in out)
+    res["batched_qps"] = round(N / tb, 1)
+    res["speedup_vs_serial"] = round(N / tb / serial_qps, 2)
+    # open-loop Poisson arrivals at 60% of the measured batched capacity
+    rate = 0.6 * N / tb
+    n2 = 1024
+    gaps = rng.exponential(1.0 / rate, n2)
+    with grus.Server(g, max_batch=64, max_inflight_batches=2) as srv:
+        t0 = time.perf_counter()
+        due = t0
+        for i in range(n2):
+            due += gaps[i]
+            while time.perf_counter() < due:
+                pass
+            srv.submit(int(kinds[i]), int(srcs[i]))
+        out = [srv.poll(300000) for _ in range(n2)]
+    lat = np.array([(r.done_us - r.submit_us) / 1e3 for r in out])
+    res.update({"poisson_rate_qps": round(rate, 1), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2), "queries": N + n2,
+                "mix": "50/50 BFS/SSSP, sources uniform over non-isolated vertices"})
+    log("serving: serial %.1f qps, batched %.1f qps (x%.2f), p50 %.1f ms p99 %.1f ms at %.0f qps" %
+        (serial_qps, N / tb, N / tb / serial_qps, res["p50_ms"], res["p99_ms"], rate))
+    g.close()
+    ctx.close()
+    return res
+
+
+def run_capped(grus, ctx):
+    """BFS + PageRank on RMAT s27: resident in HBM vs edges capped at 8 GB of HBM."""
+    res = {}
+    for mode in ("resident", "capped"):
+        pl = grus.place("capped", hbm_cap_bytes=8 << 30) if mode == "capped" else None
+        g = grus.Graph.rmat(ctx, 27, placement=pl)
+        src = g.info.max_degree_vertex
+        grus.bfs(g, src, copy=False)
+        _, st = grus.bfs(g, src, copy=False)
+        digest = g.digest()
+        _, sp = grus.pagerank(g, 0.85, 2, copy=False)
+        B = 2 * (8 * (g.V + 1) + 8 * g.E + 12 * g.V)
+        res[mode] = {"bfs_ms": round(st.device_ms, 2),
+                     "bfs_GTEPS": round(st.edges_reached / st.device_ms / 1e6, 2),
+                     "pagerank_2it_ms": round(sp.device_ms, 1),
+                     "pagerank_GBps": round(B / sp.device_ms / 1e6, 1),
+                     "hbm_graph_GB": round(g.info.device_bytes / 1e9, 2),
+                     "host_edges_GB": round(g.info.host_bytes / 1e9, 2), "bfs_digest": digest[1]}
+        if mode == "resident":
+            res["V"], res["E"] = int(g.V), int(g.E)
+        g.close()
+    res["bfs_levels_identical"] = res["resident"].pop("bfs_digest") == res["capped"].pop("bfs_digest")
+    log("capped s27: %s" % res)
+    return res
 
 
 if __name__ == "__main__":

@@ -234,6 +234,8 @@ def autotune(g, kind="bfs", srcs=(), cache=None):
     import json
     import os
     kinds = {"bfs": capi.TUNE_BFS, "sssp": capi.TUNE_SSSP, "pr": capi.TUNE_PR}
+    fields = {"bfs": ("bfs_alpha", "bfs_beta", "bu_thread_edges"), "sssp": ("sssp_delta",),
+              "pr": ("pr_relabel",)}[kind]
     sig = C.c_uint64()
     capi.check(_lib().grus_graph_signature(g.h, C.byref(sig)))
     key = "%s:%016x" % (kind, sig.value)
@@ -251,7 +253,7 @@ def autotune(g, kind="bfs", srcs=(), cache=None):
     capi.check(_lib().grus_autotune(g.h, kinds[kind], _p(srcs) if len(srcs) else None, len(srcs),
                                     C.byref(t)))
     if cache:
-        store[key] = {n: getattr(t, n) for n, _ in Tuning._fields_ if n != "reserved"}
+        store[key] = {n: getattr(t, n) for n in fields}  # only what this kind tunes
         with open(cache, "w") as f:
             json.dump(store, f, indent=1)
     return t
