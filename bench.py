@@ -53,6 +53,61 @@ def cpu_model():
     return "unknown"
 
 
+class Regions:
+    """roctx profiler regions around the timed loops.
+
+    Under `rocprofv3 --selected-regions` only the code between
+    roctxProfilerResume(0) and roctxProfilerPause(0) is profiled, so kernel
+    statistics and counters cover exactly what the JSON line times (not the
+    graph build, the autotuner's trial runs or the warm-up).  Without the
+    profiler the calls are no-ops.
+    """
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                pass
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+
+REGIONS = Regions()
+
+
+def copy_calibration():
+    """One plain device-to-device copy of exactly 1 GiB inside a profiled
+    region: FETCH_SIZE / WRITE_SIZE of this dispatch give the factors that
+    turn the counters into bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(1 << 30)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for rc in (hip.hipMalloc(ctypes.byref(a), n), hip.hipMalloc(ctypes.byref(b), n),
+               hip.hipMemset(a, 1, n), hip.hipDeviceSynchronize()):
+        if rc != 0:
+            raise RuntimeError("copy calibration: hip error %d" % rc)
+    REGIONS.resume()
+    rc = hip.hipMemcpy(b, a, n, 3)  # 3 = hipMemcpyDeviceToDevice
+    rc = rc or hip.hipDeviceSynchronize()
+    REGIONS.pause()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc != 0:
+        raise RuntimeError("copy calibration: hip error %d" % rc)
+    log("copy calibration: 1 GiB device-to-device copy done")
+
+
 def traffic(key):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled)."""
     try:
@@ -75,6 +130,10 @@ def main():
                     help="headline only (no SSSP / PageRank / CC / serving / capped)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-tune", action="store_true", help="keep the default thresholds")
+    ap.add_argument("--only", default="",
+                    help="comma list of extras to run: pagerank,cc,sssp,serving,capped")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 1 GiB device-to-device copy in a profiled region")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -84,6 +143,9 @@ def main():
     import grus_framework_amd as grus
     from oracle import oracle_py as O
 
+    REGIONS.pause()
+    if args.copy_calibration:
+        copy_calibration()
     ctx = grus.Context(0)
     t0 = time.perf_counter()
     g = grus.Graph.rmat(ctx, args.scale)
@@ -105,6 +167,7 @@ def main():
         grus.bfs(g, srcs[i % len(srcs)], copy=False)
     er = vr = 0
     dev_ms = 0.0
+    REGIONS.resume()
     t0 = time.perf_counter()
     for i in range(args.steps):
         _, st = grus.bfs(g, srcs[i % len(srcs)], copy=False)  # returns when the stream is idle
@@ -112,6 +175,7 @@ def main():
         vr += st.vertices_reached
         dev_ms += st.device_ms
     wall = time.perf_counter() - t0
+    REGIONS.pause()
     gteps = er / wall / 1e9
     algo_bytes = (4 * er + 16 * vr + 4 * g.V * args.steps) / args.steps
     achieved = algo_bytes / (dev_ms / args.steps * 1e-3) / 1e9
@@ -177,13 +241,37 @@ def main():
 
 def run_extra(grus, O, ctx, g, h, threads, args):
     out = {}
-    # ---- PageRank pull, 20 iterations, FP32 (same RMAT graph) ----
+    only = set(x for x in args.only.split(",") if x)
+
+    def want(name):
+        return not only or name in only
+
+    if want("pagerank"):
+        run_pagerank(grus, O, g, h, threads, args, out)
+    if want("cc"):
+        run_cc(grus, O, g, h, threads, args, out)
+    if want("sssp"):
+        run_sssp(grus, O, ctx, threads, args, out)
+    if want("serving"):
+        out["serving_rmat_s22"] = run_serving(grus)
+    if want("capped"):
+        try:
+            out["capped_rmat_s27"] = run_capped(grus, ctx)
+        except grus.GrusError as e:  # e.g. not enough device or host memory
+            out["capped_rmat_s27"] = {"error": str(e)}
+    return out
+
+
+def run_pagerank(grus, O, g, h, threads, args, out):
+    """PageRank pull, 20 iterations, FP32, on the headline RMAT graph."""
     grus.pagerank(g, copy=False)
     reps = 3
     ms = 0.0
+    REGIONS.resume()
     for _ in range(reps):
         _, st = grus.pagerank(g, copy=False)
         ms += st.device_ms
+    REGIONS.pause()
     ms /= reps
     per_iter = 8 * (g.V + 1) + 8 * g.E + 12 * g.V
     B = 20 * per_iter
@@ -203,7 +291,9 @@ def run_extra(grus, O, ctx, g, h, threads, args):
     out["pagerank"] = pr
     log("pagerank: %.3f ms (%.1f GB/s)" % (ms, B / ms / 1e6))
 
-    # ---- CC (hooking) on the same graph ----
+
+def run_cc(grus, O, g, h, threads, args, out):
+    """CC (hooking) on the headline RMAT graph."""
     grus.cc(g, copy=False)
     ms = min(grus.cc(g, copy=False)[1].device_ms for _ in range(3))
     cc = {"ms": round(ms, 3), "GTEPS": round(g.E / ms / 1e6, 2)}
@@ -213,7 +303,9 @@ def run_extra(grus, O, ctx, g, h, threads, args):
     out["cc"] = cc
     log("cc: %.3f ms" % ms)
 
-    # ---- SSSP on weighted RMAT s24 (w in [1,255]) ----
+
+def run_sssp(grus, O, ctx, threads, args, out):
+    """SSSP on weighted RMAT s24 (w in [1,255]) and on the 4096x4096 grid."""
     with grus.Graph.rmat(ctx, args.scale, weights=(1, 255), wseed=3) as gw:
         s = gw.info.max_degree_vertex
         grus.sssp(gw, s, copy=False)
@@ -237,13 +329,6 @@ def run_extra(grus, O, ctx, g, h, threads, args):
             grid["cpu_GTEPS"] = round(st.edges_reached / cms / 1e6, 3)
         out["sssp_grid4096"] = grid
         log("sssp grid: %.2f ms" % ms)
-
-    out["serving_rmat_s22"] = run_serving(grus)
-    try:
-        out["capped_rmat_s27"] = run_capped(grus, ctx)
-    except grus.GrusError as e:  # e.g. not enough device or host memory
-        out["capped_rmat_s27"] = {"error": str(e)}
-    return out
 
 
 def run_serving(grus):
