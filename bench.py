@@ -54,32 +54,36 @@ def cpu_model():
 
 
 class Regions:
-    """roctx profiler regions around the timed loops.
+    """Host-clock windows around the timed loops.
 
-    Under `rocprofv3 --selected-regions` only the code between
-    roctxProfilerResume(0) and roctxProfilerPause(0) is profiled, so kernel
-    statistics and counters cover exactly what the JSON line times (not the
-    graph build, the autotuner's trial runs or the warm-up).  Without the
-    profiler the calls are no-ops.
+    Each window records CLOCK_MONOTONIC and CLOCK_BOOTTIME in ns at its
+    begin and end; `--regions-out FILE` writes them as JSON.  rocprofv3
+    stamps every dispatch on the host clock, so
+    `python -m grus_framework_amd.rocprof_summary` can keep exactly the
+    dispatches of the timed loops (not the graph build, the autotuner's
+    trial runs or the warm-up) when it summarises a kernel trace or a
+    counter run of this script.
     """
 
     def __init__(self):
-        import ctypes
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                pass
+        self.windows = []
 
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
+    @staticmethod
+    def _now():
+        return {"monotonic": time.clock_gettime_ns(time.CLOCK_MONOTONIC),
+                "boottime": time.clock_gettime_ns(time.CLOCK_BOOTTIME)}
 
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
+    def begin(self, name):
+        self.windows.append({"name": name, "begin": self._now()})
+
+    def end(self, units):
+        self.windows[-1]["end"] = self._now()
+        self.windows[-1]["units"] = units
+
+    def write(self, path):
+        if path:
+            with open(path, "w") as f:
+                json.dump(self.windows, f, indent=1)
 
 
 REGIONS = Regions()
@@ -97,10 +101,10 @@ def copy_calibration():
                hip.hipMemset(a, 1, n), hip.hipDeviceSynchronize()):
         if rc != 0:
             raise RuntimeError("copy calibration: hip error %d" % rc)
-    REGIONS.resume()
+    REGIONS.begin("copy_1GiB")
     rc = hip.hipMemcpy(b, a, n, 3)  # 3 = hipMemcpyDeviceToDevice
     rc = rc or hip.hipDeviceSynchronize()
-    REGIONS.pause()
+    REGIONS.end(1)
     hip.hipFree(a)
     hip.hipFree(b)
     if rc != 0:
@@ -133,7 +137,9 @@ def main():
     ap.add_argument("--only", default="",
                     help="comma list of extras to run: pagerank,cc,sssp,serving,capped")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run a 1 GiB device-to-device copy in a profiled region")
+                    help="first run a 1 GiB device-to-device copy in its own timed window")
+    ap.add_argument("--regions-out", default="",
+                    help="write the host-clock windows of the timed loops to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -143,7 +149,6 @@ def main():
     import grus_framework_amd as grus
     from oracle import oracle_py as O
 
-    REGIONS.pause()
     if args.copy_calibration:
         copy_calibration()
     ctx = grus.Context(0)
@@ -157,7 +162,7 @@ def main():
     srcs = pick_sources(deg, 16, seed=5)
     tuned = None
     if not args.no_tune:
-        tune_srcs = pick_sources(deg, 4, seed=99)[1:]
+        tune_srcs = pick_sources(deg, 8, seed=99)[1:]
         t = grus.autotune(g, "bfs", tune_srcs)
         tuned = {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta,
                  "bu_thread_edges": t.bu_thread_edges}
@@ -167,7 +172,7 @@ def main():
         grus.bfs(g, srcs[i % len(srcs)], copy=False)
     er = vr = 0
     dev_ms = 0.0
-    REGIONS.resume()
+    REGIONS.begin("bfs_rmat_s%d_traversal" % args.scale)
     t0 = time.perf_counter()
     for i in range(args.steps):
         _, st = grus.bfs(g, srcs[i % len(srcs)], copy=False)  # returns when the stream is idle
@@ -175,7 +180,7 @@ def main():
         vr += st.vertices_reached
         dev_ms += st.device_ms
     wall = time.perf_counter() - t0
-    REGIONS.pause()
+    REGIONS.end(args.steps)
     gteps = er / wall / 1e9
     algo_bytes = (4 * er + 16 * vr + 4 * g.V * args.steps) / args.steps
     achieved = algo_bytes / (dev_ms / args.steps * 1e-3) / 1e9
@@ -236,6 +241,7 @@ def main():
     if "pagerank" in extra:
         line["roofline_pagerank"] = extra["pagerank"].pop("roofline")
     print(json.dumps(line), flush=True)
+    REGIONS.write(args.regions_out)
     ctx.close()
 
 
@@ -267,11 +273,11 @@ def run_pagerank(grus, O, g, h, threads, args, out):
     grus.pagerank(g, copy=False)
     reps = 3
     ms = 0.0
-    REGIONS.resume()
+    REGIONS.begin("pagerank_rmat_s%d_iteration" % args.scale)
     for _ in range(reps):
         _, st = grus.pagerank(g, copy=False)
         ms += st.device_ms
-    REGIONS.pause()
+    REGIONS.end(reps * 20)
     ms /= reps
     per_iter = 8 * (g.V + 1) + 8 * g.E + 12 * g.V
     B = 20 * per_iter

@@ -40,7 +40,8 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("bfs_alpha", C.c_double), ("bfs_beta", C.c_double),
                 ("bu_thread_edges", C.c_uint32), ("sssp_delta", C.c_uint32),
-                ("pr_relabel", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+                ("pr_layout", C.c_uint32), ("pr_bin_bits", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
 
 
 class GraphInfo(C.Structure):

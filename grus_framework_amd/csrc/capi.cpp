@@ -99,6 +99,9 @@ int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in) {
     need(ctx, "ctx");
     need(in, "in");
     if (in->bfs_alpha < 0 || in->bfs_beta < 0) mgg::raise(GRUS_E_INVALID, "negative threshold");
+    if (in->pr_layout > 3) mgg::raise(GRUS_E_INVALID, "pr_layout");
+    if (in->pr_bin_bits != 0 && (in->pr_bin_bits < 6 || in->pr_bin_bits > 24))
+      mgg::raise(GRUS_E_INVALID, "pr_bin_bits outside [6, 24]");
     ctx->c->tuning = *in;
   });
 }

@@ -25,6 +25,7 @@ using eid_t = uint64_t;
 constexpr uint32_t INFINIT = GRUS_INFINIT;
 constexpr int WAVE = 64;           // CDNA wavefront width
 constexpr int BLOCK = 256;         // default workgroup: 4 waves
+constexpr int NXCD = 8;            // MI355X: 8 XCDs, each with its own L2
 
 // Error carried from the engine to the C-ABI, which turns it into a status
 // code plus a thread-local message (the reference prints and exits instead,

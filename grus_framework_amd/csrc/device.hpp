@@ -12,6 +12,16 @@ namespace mgg {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// XCD-aware workgroup order.  Workgroups are dispatched round-robin over
+// the XCDs, so block b runs on XCD b % NXCD.  Giving XCD x the contiguous
+// item range [x*per, (x+1)*per) keeps the working set of neighbouring items
+// in that XCD's own L2 instead of spreading it over all eight.  gridDim.x
+// must be a multiple of NXCD; the map is a bijection on [0, gridDim.x).
+__device__ __forceinline__ uint64_t xcd_block() {
+  const uint64_t per = gridDim.x / NXCD;
+  return (uint64_t)(blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+}
+
 // number of set bits of m in lanes below this one
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -550,6 +550,9 @@ void graph_apply_placement(Graph &g, const grus_place *place) {
   if (cap && cap < fixed) raise(GRUS_E_BUDGET, "HBM cap below the offsets array size");
   uint64_t fit = cap ? (cap - fixed) / per_edge : g.E;
   uint64_t split = std::min<uint64_t>(fit, g.E);
+  // a multiple of 64 edges, so that vector column loads never straddle the
+  // HBM / host boundary and stay aligned on both sides of it
+  if (split < g.E) split &= ~63ull;
   g.split = split;
   uint64_t far = g.E - split;
   if (far) {

@@ -36,6 +36,16 @@ void scan_excl_u64(Context &ctx, const uint64_t *in, uint64_t *out, size_t n) {
                                    ctx.stream));
 }
 
+void scan_excl_u32(Context &ctx, const uint32_t *in, uint32_t *out, size_t n) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  GRUS_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, uint32_t(0), n,
+                                   rocprim::plus<uint32_t>(), ctx.stream));
+  void *t = tmp(ctx, bytes);
+  GRUS_HIP(rocprim::exclusive_scan(t, bytes, in, out, uint32_t(0), n, rocprim::plus<uint32_t>(),
+                                   ctx.stream));
+}
+
 void scan_unique_flags(Context &ctx, const uint64_t *keys, uint64_t *pos, size_t n,
                        uint64_t sentinel) {
   if (n == 0) return;

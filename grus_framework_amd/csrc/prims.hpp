@@ -7,6 +7,8 @@ namespace mgg {
 
 // out[i] = sum of in[0..i), for i in [0, n); in and out must not alias.
 void scan_excl_u64(Context &ctx, const uint64_t *in, uint64_t *out, size_t n);
+// out[i] = sum of in[0..i) in 32 bits (the caller guarantees the total fits).
+void scan_excl_u32(Context &ctx, const uint32_t *in, uint32_t *out, size_t n);
 // Exclusive scan of the 0/1 "first of its run and not the sentinel" flags of
 // sorted keys: pos[j] = number of distinct non-sentinel keys before j.
 void scan_unique_flags(Context &ctx, const uint64_t *keys, uint64_t *pos, size_t n,

@@ -92,16 +92,23 @@ typedef struct grus_tuning {
   uint32_t bu_thread_edges; /* bottom-up: edges a lane scans before handing
                                the vertex to a wave (default 32) */
   uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic) */
-  uint32_t pr_relabel;    /* PageRank on a degree-sorted relabelled copy of a
-                             symmetric HBM graph: 1 on; 0 (default) or 2 off */
-  uint32_t reserved[7];
+  uint32_t pr_layout;     /* PageRank edge layout: 0 automatic (= 2); 1 degree-
+                             sorted relabelled copy (symmetric HBM graphs);
+                             2 plain pull over the in-edges; 3 blocked
+                             (propagation blocking over bins of 2^pr_bin_bits
+                             destinations; HBM graphs with E <= 2^31, else 2).
+                             Layouts 2 and 3 give bit-identical ranks; the
+                             relabelled copy sums in another order (same
+                             ranks up to float rounding). */
+  uint32_t pr_bin_bits;   /* blocked layout bin width, log2 (0 = automatic) */
+  uint32_t reserved[6];
 } grus_tuning;
 
 /* ---- autotuning (no reference counterpart: the reference hard-codes its
  * thresholds, include/frontier.cuh:140) ----
  * Times the app on g under candidate settings (BFS: direction-switch alpha,
  * beta and the bottom-up lane/wave cut-off; SSSP: bucket width; PR:
- * relabelling) for the sample sources, keeps the fastest in the context's
+ * edge layout and bin width) for the sample sources, keeps the fastest in the context's
  * tuning and returns it.  Results never depend on the tuning. */
 #define GRUS_TUNE_BFS 0
 #define GRUS_TUNE_SSSP 1

@@ -71,7 +71,7 @@ int or_gr_read(const char *path, uint64_t *off, uint32_t *col, uint32_t *w) {
 }
 
 /* ------------------------------------------------------------------ */
-/* generators (project spec, documented in DESIGN.md)                  */
+/* generators: this project's RMAT and grid specification            */
 /* ------------------------------------------------------------------ */
 /* splitmix64 output function (Steele, Lea, Flood 2014; public domain) */
 static inline uint64_t mix64(uint64_t z) {

@@ -214,9 +214,82 @@ def test_pagerank_relabelled_path(ctx, oracle):
         t0 = ctx.tuning
         try:
             t = ctx.tuning
-            t.pr_relabel = 1
+            t.pr_layout = 1
             ctx.tuning = t
             r, _ = grus.pagerank(g)
         finally:
             ctx.tuning = t0
         assert np.abs(r - oracle.pagerank(h)).sum() <= PR_TOL
+
+
+def pagerank_with(ctx, g, layout, bits=0, iters=20):
+    t0 = ctx.tuning
+    try:
+        t = ctx.tuning
+        t.pr_layout = layout
+        t.pr_bin_bits = bits
+        ctx.tuning = t
+        return grus.pagerank(g, 0.85, iters)
+    finally:
+        ctx.tuning = t0
+
+
+@pytest.mark.parametrize("bits", [6, 10, 14])
+def test_pagerank_blocked_equals_pull(ctx, oracle, bits):
+    # the blocked layout sums the same terms in the same order as the pull:
+    # bit-identical ranks, for every bin width, symmetric and directed
+    rng = np.random.default_rng(11)
+    off, col, _ = random_directed(rng, 30000, 400000)
+    with grus.Graph.rmat(ctx, 15) as g, \
+            grus.Graph.from_csr(ctx, off, col, symmetric=False) as d:
+        for gr in (g, d):
+            a, _ = pagerank_with(ctx, gr, 2)
+            b, st = pagerank_with(ctx, gr, 3, bits)
+            print("bits %d V=%d launches pull/blocked %d" % (bits, gr.V, st.kernel_launches),
+                  flush=True)
+            assert (a.view(np.uint32) == b.view(np.uint32)).all()
+            assert st.kernel_launches == 1 + 3 * 20  # init + (scatter, pull, fixup) per iteration
+        r, _ = pagerank_with(ctx, g, 3, bits)
+        assert np.abs(r.astype(np.float64) - oracle.pagerank(host_csr(g), 0.85, 20)).sum() <= PR_TOL
+
+
+def test_pagerank_blocked_edge_cases(ctx, oracle):
+    # isolated vertices, a hub spanning every bin, rows of one edge, V not a
+    # multiple of the source chunk; iteration counts 0 and 1
+    L = 70001
+    V = L + 5
+    rows = [list(range(1, L + 1))] + [[0]] * L + [[], [V - 2], [V - 3], []]
+    off = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint64)
+    col = np.concatenate([np.array(r, np.uint32) for r in rows])
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=True) as g:
+        for it in (0, 1, 20):
+            a, _ = pagerank_with(ctx, g, 2, iters=it)
+            b, _ = pagerank_with(ctx, g, 3, 8, iters=it)
+            assert (a.view(np.uint32) == b.view(np.uint32)).all(), it
+        assert np.abs(b - oracle.pagerank(h, 0.85, 20)).sum() <= PR_TOL
+
+
+def test_pagerank_ragged_in_rows(ctx, oracle):
+    # in-edge rows built to hit every window case: runs of empty rows inside
+    # a window, rows of 1/7/8/9 edges (thread boundaries), rows of exactly a
+    # window (2048) and one edge off, rows ending on window boundaries
+    rng = np.random.default_rng(21)
+    lens = []
+    for _ in range(40):
+        lens += [0] * int(rng.integers(0, 3000))
+        lens += list(rng.choice([1, 7, 8, 9, 2047, 2048, 2049, 4096, 31, 0],
+                                int(rng.integers(1, 40))))
+    lens = np.array(lens, np.int64)
+    V = len(lens)
+    in_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    in_col = np.concatenate([np.sort(rng.choice(V, l, replace=False)) if l else
+                             np.zeros(0, np.int64) for l in lens]).astype(np.uint32)
+    out = oracle.transpose(oracle.CSR(in_off, in_col))
+    with grus.Graph.from_csr(ctx, out.off, out.col, symmetric=False) as g:
+        ref = oracle.pagerank(out, 0.85, 20, csc=oracle.transpose(out))
+        a, _ = pagerank_with(ctx, g, 2)
+        b, _ = pagerank_with(ctx, g, 3, 10)
+        print("ragged V=%d E=%d L1 %.3e" % (V, g.E, np.abs(a - ref).sum()), flush=True)
+        assert np.abs(a.astype(np.float64) - ref).sum() <= PR_TOL
+        assert (a.view(np.uint32) == b.view(np.uint32)).all()
