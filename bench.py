@@ -134,6 +134,9 @@ def main():
                     help="headline only (no SSSP / PageRank / CC / serving / capped)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-tune", action="store_true", help="keep the default thresholds")
+    ap.add_argument("--bfs-tuning", default="",
+                    help="alpha,beta,cutoff: use these BFS thresholds instead of autotuning "
+                         "(e.g. the values an unprofiled run's autotuner printed)")
     ap.add_argument("--only", default="",
                     help="comma list of extras to run: pagerank,cc,sssp,serving,capped")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -161,7 +164,14 @@ def main():
     deg = h.degrees()
     srcs = pick_sources(deg, 16, seed=5)
     tuned = None
-    if not args.no_tune:
+    if args.bfs_tuning:
+        a, b, k = args.bfs_tuning.split(",")
+        t = ctx.tuning
+        t.bfs_alpha, t.bfs_beta, t.bu_thread_edges = float(a), float(b), int(k)
+        ctx.tuning = t
+        tuned = {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta,
+                 "bu_thread_edges": t.bu_thread_edges, "source": "--bfs-tuning"}
+    elif not args.no_tune:
         tune_srcs = pick_sources(deg, 8, seed=99)[1:]
         t = grus.autotune(g, "bfs", tune_srcs)
         tuned = {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta,
@@ -225,7 +235,9 @@ def main():
         "dtype": "u32",
         "data": "synthetic (RMAT generated on the device, seed 1)",
         "config": {"workload": "bfs_rmat_s%d_ef16" % args.scale, "V": int(g.V), "E": int(h.E),
-                   "sources": len(srcs), "direction": "auto (autotuned)" if tuned else "auto",
+                   "sources": len(srcs),
+                   "direction": ("auto (" + ("pinned" if args.bfs_tuning else "autotuned") + ")")
+                   if tuned else "auto",
                    "tuning": tuned, "graph500_gteps": round(gteps / 2, 3)},
         "roofline": {"kernel": "BFS traversal of one source (all its launches; k_pull "
                                "dominates)", "bound": "hbm",

@@ -100,6 +100,7 @@ __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uin
 // so each lane keeps PULL_UNROLL independent load chains in flight; most
 // RMAT vertices find a parent at their first (lowest-id) in-neighbour.
 constexpr int PULL_UNROLL = 4;
+constexpr int PULL_BATCH = 4;  // in-edges tested per dependent load round
 __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_off,
                                                 const uint64_t *front, uint64_t *visited,
                                                 uint64_t *front_next, uint32_t *label,
@@ -145,11 +146,18 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
       if (active[j] && !found[j] && beg[j] < end[j]) {
         uint64_t e = beg[j] + 1;
         uint64_t lim = (end[j] - beg[j] > K) ? beg[j] + K : end[j];
-        for (; e < lim; e++) {
-          if (bm_test(front, edge_col(in, e))) {
-            found[j] = true;
-            break;
-          }
+        // PULL_BATCH columns, then their PULL_BATCH frontier words, are
+        // loaded independently: one column -> bitmap latency chain per batch
+        // instead of per edge (a batch may read past the parent it finds)
+        for (; e < lim && !found[j]; e += PULL_BATCH) {
+          uint32_t c[PULL_BATCH];
+#pragma unroll
+          for (int q = 0; q < PULL_BATCH; q++) c[q] = (e + q < lim) ? edge_col(in, e + q) : 0u;
+          uint64_t fw[PULL_BATCH];
+#pragma unroll
+          for (int q = 0; q < PULL_BATCH; q++) fw[q] = (e + q < lim) ? front[c[q] >> 6] : 0ull;
+#pragma unroll
+          for (int q = 0; q < PULL_BATCH; q++) found[j] = found[j] || ((fw[q] >> (c[q] & 63)) & 1ull);
         }
         is_long = !found[j] && lim < end[j];
       }
@@ -161,7 +169,8 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
       if (found[j]) {
         label[v] = next_level;
         my_found++;
-        my_edges += out_off[v + 1] - out_off[v];
+        // out-degree: the in-edge row itself on a symmetric graph
+        my_edges += (out_off == in.off) ? end[j] - beg[j] : out_off[v + 1] - out_off[v];
       }
       uint32_t slot = wave_append(&cnt->long_count, is_long);
       if (is_long) longq[slot] = v;
