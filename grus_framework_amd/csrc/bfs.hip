@@ -99,7 +99,7 @@ __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uin
 // visited words, offsets, first in-neighbours and frontier bits together,
 // so each lane keeps PULL_UNROLL independent load chains in flight; most
 // RMAT vertices find a parent at their first (lowest-id) in-neighbour.
-constexpr int PULL_UNROLL = 4;
+constexpr int PULL_UNROLL = 8;
 constexpr int PULL_BATCH = 4;  // in-edges tested per dependent load round
 __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_off,
                                                 const uint64_t *front, uint64_t *visited,

@@ -26,3 +26,60 @@ def test_cli_refuses_multi_gpu(cli):
     r = subprocess.run([os.path.join(cli, "grus_cc"), "--rmat", "8", "--ngpu", "4"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def _regions(path, windows):
+    import json
+    out = []
+    for name, b, e, units in windows:
+        out.append({"name": name, "units": units,
+                    "begin": {"monotonic": b, "boottime": b + 10 ** 12},
+                    "end": {"monotonic": e, "boottime": e + 10 ** 12}})
+    path.write_text(json.dumps(out))
+
+
+def test_rocprof_summary_keeps_only_timed_windows(tmp_path):
+    # dispatches inside [1000, 2000] belong to the window; the ones before
+    # (a build) and after (another phase) must not be counted
+    from grus_framework_amd import rocprof_summary as rs
+    _regions(tmp_path / "r.json", [("bfs", 1000, 2000, 2)])
+    rows = ["Kernel_Name,Start_Timestamp,End_Timestamp",
+            "k_build(int),500,900",
+            "k_pull(int),1100,1400",
+            "k_pull(int),1500,1800",
+            "k_push(int),1900,1950",
+            "k_other(int),2100,2500"]
+    (tmp_path / "t.csv").write_text("\n".join(rows) + "\n")
+    regions = rs.load_regions(str(tmp_path / "r.json"))
+    trows = rs.load_rows(str(tmp_path / "t.csv"))
+    clock = rs.pick_clock(regions, [int(r["Start_Timestamp"]) for r in trows])
+    assert clock == "monotonic"
+    text = rs.summarise_trace(regions, trows, clock)
+    assert "k_build" not in text and "k_other" not in text
+    # 2 units, 3 dispatches, (300 + 300 + 50) ns = 0.00065 ms in total
+    assert "(2 units, 3 dispatches, 0.0003 ms kernel time per unit)" in text
+    assert "| `k_pull` | 2 | 1.00 | 0.001 | 0.30 | 0.0003 | 92.3% |" in text
+
+
+def test_rocprof_summary_counter_calibration(tmp_path):
+    import json
+    from grus_framework_amd import rocprof_summary as rs
+    # the copy window reads 512 KiB per MiB copied -> factor 2; the timed
+    # window then holds 100 + 300 KiB over 4 units -> 100 KiB x 1024 x 2
+    _regions(tmp_path / "r.json", [("copy_1GiB", 0, 100, 1), ("pr_iter", 200, 400, 4)])
+    rows = ["Kernel_Name,Counter_Name,Counter_Value,Start_Timestamp,End_Timestamp",
+            "copy,FETCH_SIZE,524288,10,20",
+            "k_pull_window(x),FETCH_SIZE,100,210,220",
+            "k_pull_window(x),FETCH_SIZE,300,300,320",
+            "k_late(x),FETCH_SIZE,999999,500,520"]
+    (tmp_path / "c.csv").write_text("\n".join(rows) + "\n")
+    regions = rs.load_regions(str(tmp_path / "r.json"))
+    crow = rs.load_rows(str(tmp_path / "c.csv"))
+    out = tmp_path / "traffic.json"
+    out.write_text(json.dumps({"pr_iter": {"write_bytes": 7}}))
+    text = rs.summarise_counters(regions, crow, "monotonic", str(out))
+    assert "bytes = KiB x 1024 x 2.0000" in text
+    t = json.loads(out.read_text())
+    assert t["pr_iter"]["fetch_bytes"] == 100 * 1024 * 2
+    assert t["pr_iter"]["write_bytes"] == 7  # merged, not replaced
+    assert "k_late" not in text
