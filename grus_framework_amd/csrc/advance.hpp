@@ -129,8 +129,9 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
         }
         uint64_t e = row + (k - base);
         dst = edge_col(g, e);
-        bool upd = job.update(u, dst, e, edge_w(g, e));
-        gen = job.generate(upd, dst);
+        uint32_t val = 0;  // what the updater wrote (a distance), for the generator
+        bool upd = job.update(u, dst, e, edge_w(g, e), &val);
+        gen = job.generate(upd, dst, val);
       }
       uint32_t slot = block_append(&cnt->next_count, gen == 1, s_app);
       if (gen == 1) {

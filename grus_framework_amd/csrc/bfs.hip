@@ -65,14 +65,16 @@ struct job_t {
   uint32_t next_level;  // job.itr + 1
   // updater (bfs.cu:51-60): first discovery sets the level; the bitmap
   // claim makes the check-then-store race-free
-  __device__ __forceinline__ bool update(vtx_t, vtx_t dst, eid_t, weight_t) const {
+  __device__ __forceinline__ bool update(vtx_t, vtx_t dst, eid_t, weight_t, uint32_t *) const {
     if (bm_test(visited, dst)) return false;
     if (!bm_claim(visited, dst)) return false;
     label[dst] = next_level;
     return true;
   }
   // generator (bfs.cu:67-71): a claimed vertex joins the next frontier once
-  __device__ __forceinline__ int generate(bool updated, vtx_t) const { return updated ? 1 : 0; }
+  __device__ __forceinline__ int generate(bool updated, vtx_t, uint32_t) const {
+    return updated ? 1 : 0;
+  }
   __device__ __forceinline__ uint64_t out_degree(vtx_t v) const {
     return out_off[v + 1] - out_off[v];
   }

@@ -114,14 +114,16 @@ struct job_t {
   const uint64_t *out_off;
   uint32_t stamp_id;
   // updater cc.cu:69-79
-  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t) const {
+  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t, uint32_t *) const {
     uint32_t ls = ld(&label[src]);
     if (ld(&label[dst]) <= ls) return false;
     return atomicMin(&label[dst], ls) > ls;
   }
   // generator cc.cu:86-91 (the byte CAS flag becomes a round stamp)
-  __device__ __forceinline__ int generate(bool updated, vtx_t dst) const {
-    return (updated && atomicExch(&stamp[dst], stamp_id) != stamp_id) ? 1 : 0;
+  __device__ __forceinline__ int generate(bool updated, vtx_t dst, uint32_t) const {
+    // stamps are unique per round: a plain read showing this round's is exact
+    if (!updated || stamp[dst] == stamp_id) return 0;
+    return atomicExch(&stamp[dst], stamp_id) != stamp_id ? 1 : 0;
   }
   __device__ __forceinline__ uint64_t out_degree(vtx_t v) const {
     return out_off[v + 1] - out_off[v];

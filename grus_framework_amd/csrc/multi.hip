@@ -24,7 +24,9 @@ namespace mgg {
 namespace ms {
 
 constexpr uint32_t K_PULL = 32;
-constexpr uint32_t CHUNK = 64;
+constexpr uint32_t CHUNK = 64;       // SSSP relax: out-edges per wave task (one per lane)
+constexpr int SPARSE_LANES = 16;     // at most this many active queries: edge-parallel relax (measured best of 4..64)
+static_assert(CHUNK == WAVE, "the edge-parallel relax covers a chunk with one edge per lane");
 
 struct LaneStart {
   uint32_t s[64];
@@ -147,12 +149,12 @@ struct push_job_t {
   static constexpr bool kFar = false;
   const uint64_t *front, *visit;
   uint64_t *next;
-  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t) const {
+  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t, uint32_t *) const {
     uint64_t m = front[src] & ~visit[dst];
     if (m && (next[dst] & m) != m) atomicOr((unsigned long long *)&next[dst], m);
     return false;
   }
-  __device__ __forceinline__ int generate(bool, vtx_t) const { return 0; }
+  __device__ __forceinline__ int generate(bool, vtx_t, uint32_t) const { return 0; }
   __device__ __forceinline__ uint64_t out_degree(vtx_t) const { return 0; }
 };
 
@@ -233,15 +235,39 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t 
     uint64_t e1 = g.off[u + 1];
     if (e1 > e + CHUNK) e1 = e + CHUNK;
     const uint64_t fm = front[u];
+    if (__popcll(fm) <= SPARSE_LANES) {
+      // few queries active at u: lanes take CHUNK different edges at once
+      // and loop over the active query bits (wave-uniform branch)
+      const uint64_t ee = e + lane;
+      const bool valid = ee < e1;
+      const uint32_t v = valid ? edge_col(g, ee) : 0u;
+      const uint32_t w = valid ? edge_w(g, ee) : 0u;
+      uint64_t imp = 0;
+      for (uint64_t m = fm; m; m &= m - 1) {
+        const uint32_t qb = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+        const uint32_t du = dist[(uint64_t)u * 64 + qb];
+        const uint64_t nd = (uint64_t)du + w;
+        if (valid && du < INFINIT && nd < INFINIT) {
+          uint32_t *dv = &dist[(uint64_t)v * 64 + qb];
+          if (nd < *dv && atomicMin(dv, (uint32_t)nd) > nd) imp |= 1ull << qb;
+        }
+      }
+      if (imp && (next[v] & imp) != imp) atomicOr((unsigned long long *)&next[v], imp);
+      continue;
+    }
     const bool act = (fm >> lane) & 1ull;
     const uint32_t du = act ? dist[(uint64_t)u * 64 + lane] : INFINIT;
     for (; e < e1; e++) {
       uint32_t v = edge_col(g, e);
       uint64_t nd = (uint64_t)du + edge_w(g, e);
       bool imp = false;
-      if (act && nd < INFINIT) imp = atomicMin(&dist[(uint64_t)v * 64 + lane], (uint32_t)nd) > nd;
+      // distances only decrease and next[] bits are only set within this
+      // kernel: a plain (possibly stale) read that already shows no
+      // improvement, or the bits set, makes the device-scope atomic moot
+      uint32_t *dv = &dist[(uint64_t)v * 64 + lane];
+      if (act && nd < INFINIT && nd < *dv) imp = atomicMin(dv, (uint32_t)nd) > nd;
       uint64_t m = __ballot(imp);
-      if (m && lane == 0) atomicOr((unsigned long long *)&next[v], m);
+      if (m && lane == 0 && (next[v] & m) != m) atomicOr((unsigned long long *)&next[v], m);
     }
   }
 }
@@ -309,6 +335,63 @@ __global__ void k_digest(const uint32_t *row, uint64_t V, unsigned long long *ou
   }
 }
 
+// digests of the finished BFS lanes (rows of the row-major [lanes][V]
+// state): blockIdx.y picks the k-th lane of the mask
+__global__ void k_digest_rows(const uint32_t *vals, uint64_t V, uint64_t mask,
+                              unsigned long long *out) {
+  uint64_t m = mask;
+  for (uint32_t k = 0; k < blockIdx.y; k++) m &= m - 1;  // drop the lowest set bits
+  if (!m) return;
+  const uint32_t lane = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+  const uint32_t *row = vals + (uint64_t)lane * V;
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long h = 0, n = 0;
+  for (; i < V; i += stride) {
+    const uint32_t x = row[i];
+    h += digest_term(i, x);
+    n += (x != INFINIT);
+  }
+  h = wave_sum(h);
+  n = wave_sum(n);
+  if (lane_id() == 0) {
+    atomicAdd(&out[2 * lane], h);
+    atomicAdd(&out[2 * lane + 1], n);
+  }
+}
+
+// digests of the finished SSSP lanes in one sweep of the vertex-major
+// [V][64] state: lane l of every wave reads column l (coalesced 256-byte
+// rows) and sums its own lane's terms
+__global__ __launch_bounds__(BLOCK) void k_digest_cols(const uint32_t *vals, uint64_t V,
+                                                       uint64_t mask, unsigned long long *out) {
+  constexpr int NW = BLOCK / WAVE;
+  __shared__ unsigned long long s_h[NW][WAVE], s_n[NW][WAVE];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const bool mine = (mask >> lane) & 1ull;
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  unsigned long long h = 0, n = 0;
+  if (mine) {
+    for (uint64_t v = wave; v < V; v += nwaves) {
+      const uint32_t x = vals[v * 64 + lane];
+      h += digest_term(v, x);
+      n += (x != INFINIT);
+    }
+  }
+  s_h[wid][lane] = h;
+  s_n[wid][lane] = n;
+  __syncthreads();
+  if (wid == 0 && mine) {
+    for (int w = 1; w < NW; w++) {
+      h += s_h[w][lane];
+      n += s_n[w][lane];
+    }
+    atomicAdd(&out[2 * lane], h);
+    atomicAdd(&out[2 * lane + 1], n);
+  }
+}
+
 }  // namespace ms
 
 void row_digest(Context &ctx, const uint32_t *row, uint64_t V, uint64_t *reached,
@@ -341,7 +424,7 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
   queue_.alloc((V + 64) * 4);
   longq_.alloc((V + 1) * 4);
   row_.alloc((V + 1) * 4);
-  digest_.alloc(64);
+  digest_.alloc(64 * 16);
   srcs_dev_.alloc(1024);
   cnt_.alloc(sizeof(MsCounters));
   push_cnt_.alloc(sizeof(StepCounters));
@@ -546,6 +629,31 @@ void MsEngine::extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t
     GRUS_HIP(hipMemcpyAsync(host_out, row, V * 4, hipMemcpyDeviceToHost, ctx.stream));
     ctx.sync();
   }
+}
+
+void MsEngine::digest(uint64_t mask, uint64_t *reached, uint64_t *checksum) {
+  if (!mask) return;
+  Context &ctx = *ctx_;
+  ctx.activate();
+  const uint64_t V = g_.V;
+  auto *d = digest_.as<unsigned long long>();
+  GRUS_HIP(hipMemsetAsync(d, 0, 64 * 16, ctx.stream));
+  if (kind_ == MsKind::BFS) {
+    dim3 grid(grid_cap(ctx, V, BLOCK, 2), (unsigned)__builtin_popcountll(mask));
+    ms::k_digest_rows<<<grid, BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V, mask, d);
+  } else {
+    ms::k_digest_cols<<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
+        vals_.as<uint32_t>(), V, mask, d);
+  }
+  GRUS_LAUNCH_CHECK();
+  unsigned long long h[128];
+  GRUS_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  for (int l = 0; l < 64; l++)
+    if ((mask >> l) & 1ull) {
+      checksum[l] = h[2 * l];
+      reached[l] = h[2 * l + 1];
+    }
 }
 
 void MsEngine::release(int lane) {

@@ -46,9 +46,12 @@ class MsEngine {
   // one expansion step of every busy lane; returns lanes that finished
   std::vector<int> step();
   // lane result (V u32) into a device row, then optionally to host memory;
-  // reached = finite entries, checksum = FNV-1a 64 of the V u32 values
+  // reached = finite entries, checksum = sum over v of mix64(v << 32 | value)
   void extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t *checksum,
                uint32_t *device_out = nullptr);
+  // reached / checksum (as extract) of every lane in mask, in one pass over
+  // the engine's state; results land at [lane]
+  void digest(uint64_t mask, uint64_t *reached, uint64_t *checksum);
   void release(int lane);  // lane becomes free (reset lazily at next admit)
   Context &ctx() { return *ctx_; }
   uint64_t steps() const { return t_; }
@@ -65,7 +68,6 @@ class MsEngine {
   uint64_t dirty_ = 0;     // lanes whose bits must be cleared before reuse
   uint32_t start_[64] = {};
   uint64_t t_ = 0;         // global step counter
-  bool pull_ = false;
   uint64_t qn_ = 0, front_edges_ = 0;
   Buf visit_, front_, next_, vals_, queue_, longq_, row_, digest_, srcs_dev_, cnt_, push_cnt_;
   PushWork push_;

@@ -150,12 +150,17 @@ class ServerImpl : public Server {
           }
         }
         if (!eng.active()) continue;
-        for (int lane : eng.step()) {
+        const std::vector<int> fin = eng.step();
+        if (fin.empty()) continue;
+        // one digest pass for every lane that finished in this step
+        uint64_t mask = 0, reached[64], checksum[64];
+        for (int lane : fin) mask |= 1ull << lane;
+        eng.digest(mask, reached, checksum);
+        for (int lane : fin) {
           const Query &q = lane_q[lane];
-          uint64_t reached = 0, checksum = 0;
-          eng.extract(lane, opts_.copy_results ? q.out : nullptr, &reached, &checksum);
+          if (opts_.copy_results) eng.extract(lane, q.out, nullptr, nullptr);
           eng.release(lane);
-          finish(q, GRUS_OK, reached, checksum);
+          finish(q, GRUS_OK, reached[lane], checksum[lane]);
         }
       }
     } catch (const Error &e) {

@@ -24,6 +24,7 @@ namespace mgg {
 
 struct SsspWork {
   Buf q[2], far[2], near_stamp, far_stamp, pctl;
+  Buf front_bm, longq;  // pull rounds: frontier bitmap, rows handed to waves
   PushWork push;
   uint64_t cap_v = 0;
   uint32_t stamp = 0;  // last stamp handed out; stamps only grow
@@ -36,6 +37,8 @@ struct SsspWork {
     far[1].alloc((V + 1) * 4);
     near_stamp.alloc((V + 1) * 4);
     far_stamp.alloc((V + 1) * 4);
+    front_bm.alloc((V / 64 + 2) * 8);
+    longq.alloc((V + 1) * 4);
     fill_u32(ctx, near_stamp.as<uint32_t>(), V + 1, 0);
     fill_u32(ctx, far_stamp.as<uint32_t>(), V + 1, 0);
     push.ensure(V);
@@ -64,24 +67,147 @@ struct job_t {
   uint32_t T;           // near threshold (exclusive)
   uint32_t near_id;     // stamp of the near queue being built
   uint32_t far_id;      // stamp of the current far epoch
-  // updater (sssp.cu:51-60) with an atomic min
-  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t w) const {
-    uint64_t nd = (uint64_t)__atomic_load_n(&dist[src], __ATOMIC_RELAXED) + w;
-    if (nd >= INFINIT) return false;  // distances are u32 below INFINIT
+  // updater (sssp.cu:51-60) with an atomic min.  Distances only decrease,
+  // so plain (cached) reads can only be stale-high: relaxing with a stale
+  // dist[src] can only miss an improvement whose own update re-queues src,
+  // and a stale dist[dst] only lets through an atomic that turns out not to
+  // improve.  The fixed point -- and the result -- are unchanged, and most
+  // non-improving edges cost no device-scope atomic.
+  __device__ __forceinline__ bool update(vtx_t src, vtx_t dst, eid_t, weight_t w,
+                                         uint32_t *nd_out) const {
+    const uint64_t nd = (uint64_t)dist[src] + w;
+    if (nd >= INFINIT || nd >= dist[dst]) return false;  // distances are u32 below INFINIT
+    *nd_out = (uint32_t)nd;
     return atomicMin(&dist[dst], (uint32_t)nd) > (uint32_t)nd;
   }
   // generator: an improved vertex joins the near queue (below T) or the far
-  // queue, once per queue
-  __device__ __forceinline__ int generate(bool updated, vtx_t dst) const {
+  // queue, once per queue.  nd is the distance this thread wrote; a thread
+  // that lowered it further classifies it again itself.  Stamps are unique
+  // per queue, so a plain read that already shows this queue's stamp is
+  // exact.
+  __device__ __forceinline__ int generate(bool updated, vtx_t dst, uint32_t nd) const {
     if (!updated) return 0;
-    uint32_t d = __atomic_load_n(&dist[dst], __ATOMIC_RELAXED);
-    if (d < T) return atomicExch(&near_stamp[dst], near_id) != near_id ? 1 : 0;
+    if (nd < T) {
+      if (near_stamp[dst] == near_id) return 0;
+      return atomicExch(&near_stamp[dst], near_id) != near_id ? 1 : 0;
+    }
+    if (far_stamp[dst] == far_id) return 0;
     return atomicExch(&far_stamp[dst], far_id) != far_id ? 2 : 0;
   }
   __device__ __forceinline__ uint64_t out_degree(vtx_t v) const {
     return out_off[v + 1] - out_off[v];
   }
 };
+
+// ---- pull rounds (large frontiers) ----
+// A vertex whose distance is below T_lo (the threshold of the previous
+// epoch) is final and skipped.  Every other vertex takes the minimum of
+// dist[u] + w over its in-neighbours u in the frontier; only its own thread
+// writes dist[v], so no atomics are needed on distances.  Reads of dist[u]
+// may see values lowered in this same round: still lengths of real paths.
+constexpr uint32_t PULL_K = 32;  // rows longer than this go to a wave
+
+// classify an improved vertex: near queue below T, else the far list once
+// per epoch (its owner is the only writer of both stamps this round)
+__device__ __forceinline__ void pull_classify(uint32_t v, uint32_t best, uint32_t T,
+                                              uint32_t *near_stamp, uint32_t near_id,
+                                              uint32_t *far_stamp, uint32_t far_id, bool *near,
+                                              bool *far) {
+  if (best < T) {
+    *near = true;
+    near_stamp[v] = near_id;
+  } else if (far_stamp[v] != far_id) {
+    far_stamp[v] = far_id;
+    *far = true;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_pull_relax(CsrView in, const uint64_t *front,
+                                                      uint32_t *dist, uint32_t T_lo, uint32_t T,
+                                                      uint32_t *near_stamp, uint32_t near_id,
+                                                      uint32_t *far_stamp, uint32_t far_id,
+                                                      uint32_t *qn, uint32_t *qfar,
+                                                      uint32_t *longq, StepCounters *cnt,
+                                                      const uint64_t *out_off) {
+  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
+  const uint64_t V = in.V;
+  uint64_t my_edges = 0;
+  // block-uniform loop: block_append needs every thread of the workgroup
+  for (uint64_t b0 = blockIdx.x * (uint64_t)BLOCK; b0 < V; b0 += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t v = b0 + threadIdx.x;
+    bool is_long = false, near = false, far = false;
+    if (v < V) {
+      const uint32_t d = dist[v];
+      const uint64_t a = in.off[v], b = in.off[v + 1];
+      if (d >= T_lo && b > a) {
+        if (b - a > PULL_K) {
+          is_long = true;
+        } else {
+          uint32_t best = d;
+          for (uint64_t e = a; e < b; e++) {
+            const uint32_t u = edge_col(in, e);
+            if (bm_test(front, u)) {
+              const uint64_t c = (uint64_t)dist[u] + edge_w(in, e);
+              if (c < best) best = (uint32_t)c;
+            }
+          }
+          if (best < d) {
+            dist[v] = best;
+            pull_classify((uint32_t)v, best, T, near_stamp, near_id, far_stamp, far_id, &near,
+                          &far);
+          }
+        }
+      }
+    }
+    const uint32_t sn = block_append(&cnt->next_count, near, s_app);
+    if (near) {
+      qn[sn] = (uint32_t)v;
+      my_edges += out_off[v + 1] - out_off[v];
+    }
+    const uint32_t sf = block_append(&cnt->far_count, far, s_app);
+    if (far) qfar[sf] = (uint32_t)v;
+    const uint32_t sl = block_append(&cnt->long_count, is_long, s_app);
+    if (is_long) longq[sl] = (uint32_t)v;
+  }
+  my_edges = wave_sum(my_edges);
+  if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+}
+
+// rows longer than PULL_K: one wave per vertex, lanes stride its in-edges
+__global__ __launch_bounds__(BLOCK) void k_pull_relax_long(
+    CsrView in, const uint64_t *front, uint32_t *dist, uint32_t T, uint32_t *near_stamp,
+    uint32_t near_id, uint32_t *far_stamp, uint32_t far_id, uint32_t *qn, uint32_t *qfar,
+    const uint32_t *longq, StepCounters *cnt, const uint64_t *out_off) {
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint32_t lane = lane_id();
+  const uint32_t n = cnt->long_count;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    const uint32_t v = longq[i];
+    const uint32_t d = dist[v];
+    const uint64_t a = in.off[v], b = in.off[v + 1];
+    uint32_t best = d;
+    for (uint64_t e = a + lane; e < b; e += WAVE) {
+      const uint32_t u = edge_col(in, e);
+      if (bm_test(front, u)) {
+        const uint64_t c = (uint64_t)dist[u] + edge_w(in, e);
+        if (c < best) best = (uint32_t)c;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor(best, o));
+    if (lane == 0 && best < d) {
+      dist[v] = best;
+      bool near = false, far = false;
+      pull_classify(v, best, T, near_stamp, near_id, far_stamp, far_id, &near, &far);
+      if (near) {
+        qn[atomicAdd(&cnt->next_count, 1u)] = v;
+        atomicAdd((unsigned long long *)&cnt->next_edges, out_off[v + 1] - out_off[v]);
+      }
+      if (far) qfar[atomicAdd(&cnt->far_count, 1u)] = v;
+    }
+  }
+}
 
 __global__ void k_init(uint32_t *dist, uint64_t V, uint32_t src, uint32_t *q0) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -417,7 +543,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     GRUS_HIP(hipMemcpy(o, g.off.as<uint64_t>() + src, 16, hipMemcpyDeviceToHost));
     src_deg = o[1] - o[0];
   }
-  uint64_t launches = 0, rounds = 0, relaxed_vertices = 0;
+  uint64_t launches = 0, rounds = 0, relaxed_vertices = 0, pull_rounds = 0;
   // low-degree graphs (grids, road-like): one persistent cooperative launch
   static const bool no_persist = getenv("GRUS_NO_PERSISTENT") != nullptr;
   const bool persistent = g.max_deg <= 32 && !no_persist;
@@ -433,24 +559,56 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   int cur = 0, fcur = 0;
   uint64_t nn = persistent ? 0 : 1, mf = src_deg, nfar = 0;
   uint64_t T = delta;
+  uint64_t T_lo = 0;  // distances below it are final (threshold of the previous epoch)
   uint32_t far_id = w.next_stamp(ctx, V);
   const uint64_t round_cap = 64 * V + 1024;
+  // a round pulls when its frontier's out-edges exceed E / pull_div.  The
+  // automatic setting never pulls: on RMAT s24 a pull round (every unsettled
+  // vertex scans all its in-edges, no early exit) costs ~10 ms, more than
+  // the atomic-filtered push of the same frontier
+  const uint32_t pull_div = ctx.tuning.sssp_pull_div;
+  const uint64_t pull_edges =
+      (pull_div == 0 || pull_div == 0xFFFFFFFFu) ? ~0ull : g.E / pull_div;
+  const bool can_pull = g.symmetric || g.placement != GRUS_PLACE_CAPPED;  // needs the in-edges
+  const uint64_t W = V / 64 + 1;
   for (;;) {
     while (nn) {
       if (++rounds > round_cap) raise(GRUS_E_HIP, "sssp did not converge");
       relaxed_vertices += nn;
       GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
-      push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nn);
-      sssp::job_t job{dist,
-                      w.near_stamp.as<uint32_t>(),
-                      w.far_stamp.as<uint32_t>(),
-                      out.off,
-                      (uint32_t)std::min<uint64_t>(T, 0xFFFFFFFFull),
-                      w.next_stamp(ctx, V),
-                      far_id};
-      push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nn, mf, job,
-                   w.q[cur ^ 1].as<uint32_t>(), cnt, w.far[fcur].as<uint32_t>() + nfar);
-      launches += 4;
+      const uint32_t Tk = (uint32_t)std::min<uint64_t>(T, 0xFFFFFFFFull);
+      const uint32_t near_id = w.next_stamp(ctx, V);
+      if (can_pull && mf > pull_edges) {
+        const CsrView in = g.csc();
+        uint64_t *fbm = w.front_bm.as<uint64_t>();
+        fill_u64(ctx, fbm, W, 0);
+        queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nn, fbm, W);
+        sssp::k_pull_relax<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+            in, fbm, dist, (uint32_t)std::min<uint64_t>(T_lo, 0xFFFFFFFFull), Tk,
+            w.near_stamp.as<uint32_t>(), near_id, w.far_stamp.as<uint32_t>(), far_id,
+            w.q[cur ^ 1].as<uint32_t>(), w.far[fcur].as<uint32_t>() + nfar,
+            w.longq.as<uint32_t>(), cnt, out.off);
+        GRUS_LAUNCH_CHECK();
+        sssp::k_pull_relax_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+            in, fbm, dist, Tk, w.near_stamp.as<uint32_t>(), near_id, w.far_stamp.as<uint32_t>(),
+            far_id, w.q[cur ^ 1].as<uint32_t>(), w.far[fcur].as<uint32_t>() + nfar,
+            w.longq.as<uint32_t>(), cnt, out.off);
+        GRUS_LAUNCH_CHECK();
+        launches += 4;
+        pull_rounds++;
+      } else {
+        push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nn);
+        sssp::job_t job{dist,
+                        w.near_stamp.as<uint32_t>(),
+                        w.far_stamp.as<uint32_t>(),
+                        out.off,
+                        Tk,
+                        near_id,
+                        far_id};
+        push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nn, mf, job,
+                     w.q[cur ^ 1].as<uint32_t>(), cnt, w.far[fcur].as<uint32_t>() + nfar);
+        launches += 4;
+      }
       ctx.read_ctl(3);
       nn = hc->next_count;
       mf = hc->next_edges;
@@ -484,6 +642,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     mf = hc->next_edges;
     nfar = hc->far_count;
     fcur ^= 1;
+    T_lo = T;
     T = T_new;
   }
   GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
@@ -495,6 +654,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     reached_stats(ctx, out.off, dist, V, &st->vertices_reached, &st->edges_reached);
     st->device_ms = ms;
     st->iterations = (uint32_t)rounds;
+    st->pull_iterations = (uint32_t)pull_rounds;
     st->kernel_launches = launches;
     st->reserved[0] = relaxed_vertices;
     st->reserved[1] = delta;

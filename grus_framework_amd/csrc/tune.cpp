@@ -7,7 +7,7 @@
 //   BFS   coordinate descent over alpha (push->pull: m_f > m_u/alpha), beta
 //         (pull->push: n_f < V/beta) and the bottom-up lane/wave cut-off
 //         (edges a lane scans before handing a vertex to a wave)
-//   SSSP  bucket width delta, as multiples of the mean edge weight
+//   SSSP  bucket width delta, and the frontier size at which a round pulls
 //   PR    edge layout (blocked / plain pull) and the blocked bin width
 // Every candidate computes the same result (the settings only change the
 // schedule), so tuning never changes answers.  The winner is written into
@@ -88,6 +88,8 @@ void autotune(Graph &g, int kind, const uint32_t *srcs, int n, grus_tuning *out)
     for (uint32_t d : {32u, 64u, 128u, 256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u, 65536u})
       cands.push_back(d);
     sweep<uint32_t>(g, kind, s, &grus_tuning::sssp_delta, cands, &best);
+    sweep<uint32_t>(g, kind, s, &grus_tuning::sssp_pull_div, {4, 8, 16, 64, 128, 0xFFFFFFFFu},
+                    &best);
   } else {
     // the relabelled layout is left out: it only pays on graphs whose
     // numbering scatters the hubs, and it costs a full relabelling build

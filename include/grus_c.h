@@ -101,7 +101,12 @@ typedef struct grus_tuning {
                              relabelled copy sums in another order (same
                              ranks up to float rounding). */
   uint32_t pr_bin_bits;   /* blocked layout bin width, log2 (0 = automatic) */
-  uint32_t reserved[6];
+  uint32_t sssp_pull_div; /* SSSP: a round pulls (each unsettled vertex takes
+                             the min over its in-neighbours in the frontier)
+                             when the frontier's out-edges exceed
+                             E / sssp_pull_div; 0 = automatic (push
+                             only), 0xFFFFFFFF = push only */
+  uint32_t reserved[5];
 } grus_tuning;
 
 /* ---- autotuning (no reference counterpart: the reference hard-codes its

@@ -293,3 +293,38 @@ def test_pagerank_ragged_in_rows(ctx, oracle):
         print("ragged V=%d E=%d L1 %.3e" % (V, g.E, np.abs(a - ref).sum()), flush=True)
         assert np.abs(a.astype(np.float64) - ref).sum() <= PR_TOL
         assert (a.view(np.uint32) == b.view(np.uint32)).all()
+
+
+def sssp_with(ctx, g, src, pull_div):
+    t0 = ctx.tuning
+    try:
+        t = ctx.tuning
+        t.sssp_pull_div = pull_div
+        ctx.tuning = t
+        return grus.sssp(g, src)
+    finally:
+        ctx.tuning = t0
+
+
+ALWAYS_PULL, NEVER_PULL = 0xFFFFFFFE, 0xFFFFFFFF
+
+
+def test_sssp_pull_rounds_match_push(ctx, oracle):
+    # every round pulled, every round pushed, and the automatic mix all give
+    # Dijkstra's distances: symmetric weighted RMAT and a directed graph
+    # (whose pull rounds read the transpose with its weights)
+    rng = np.random.default_rng(8)
+    off, col, w = random_directed(rng, 5000, 300000, wmax=1000)  # max degree > 32: host rounds
+    with grus.Graph.rmat(ctx, 14, weights=(1, 255), wseed=3) as g, \
+            grus.Graph.from_csr(ctx, off, col, w, symmetric=False) as d:
+        for gr, src in ((g, g.info.max_degree_vertex), (d, 7)):
+            ref = oracle.sssp(host_csr(gr), src)
+            for div in (ALWAYS_PULL, NEVER_PULL, 32):
+                dist, st = sssp_with(ctx, gr, src, div)
+                print("V=%d div=%x rounds %d pull %d" % (gr.V, div, st.iterations,
+                                                          st.pull_iterations), flush=True)
+                assert (dist == ref).all()
+                if div == ALWAYS_PULL:  # a queue of out-degree-0 vertices still pushes
+                    assert st.pull_iterations >= 1
+                if div == NEVER_PULL:
+                    assert st.pull_iterations == 0
