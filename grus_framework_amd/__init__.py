@@ -235,7 +235,7 @@ def autotune(g, kind="bfs", srcs=(), cache=None):
     import os
     kinds = {"bfs": capi.TUNE_BFS, "sssp": capi.TUNE_SSSP, "pr": capi.TUNE_PR}
     fields = {"bfs": ("bfs_alpha", "bfs_beta", "bu_thread_edges"),
-              "sssp": ("sssp_delta", "sssp_pull_div"),
+              "sssp": ("sssp_delta", "sssp_pull_div", "sssp_local_hops"),
               "pr": ("pr_layout", "pr_bin_bits")}[kind]
     sig = C.c_uint64()
     capi.check(_lib().grus_graph_signature(g.h, C.byref(sig)))

@@ -41,7 +41,8 @@ class Tuning(C.Structure):
     _fields_ = [("bfs_alpha", C.c_double), ("bfs_beta", C.c_double),
                 ("bu_thread_edges", C.c_uint32), ("sssp_delta", C.c_uint32),
                 ("pr_layout", C.c_uint32), ("pr_bin_bits", C.c_uint32),
-                ("sssp_pull_div", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("sssp_pull_div", C.c_uint32), ("sssp_local_hops", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
 
 
 class GraphInfo(C.Structure):

@@ -300,6 +300,7 @@ struct PParams {
   PCtl *ctl;
   uint64_t delta, round_cap, timeout;
   uint32_t stamp0, max_deg;
+  uint32_t local_hops;  // extra in-workgroup relaxation passes per round
 };
 
 __device__ __forceinline__ void clear_slot(PCtl *c, uint32_t step) {
@@ -337,7 +338,7 @@ __device__ __forceinline__ void publish(const uint32_t *lq, uint32_t lcount, uin
 
 __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
   __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP];
-  __shared__ uint32_t s_nn, s_nf, s_base;
+  __shared__ uint32_t s_nn, s_nf, s_base, s_hi;
   const CsrView g = p.g;
   unsigned int gen = 0;
   uint32_t step = 0, stamp = p.stamp0;
@@ -361,8 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       const uint32_t Tk = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
       if (threadIdx.x == 0) s_nn = s_nf = 0;
       __syncthreads();
-      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride) {
-        const uint32_t u = ld_agent(&q[cur][i]);
+      auto relax = [&](uint32_t u) {
         const uint64_t e0 = g.off[u], e1 = g.off[u + 1];
         const uint32_t du = ld_agent(&p.dist[u]);
         for (uint64_t e = e0; e < e1; e++) {
@@ -377,8 +377,27 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
             stage(v, s_far, &s_nf, &C[1], far[fcur], nfar);
           }
         }
-      }
+      };
+      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride)
+        relax(ld_agent(&q[cur][i]));
       __syncthreads();
+      // local hops: the workgroup relaxes again the near vertices it has just
+      // generated (held in LDS), so the frontier advances several hops per
+      // grid barrier.  Everything staged is still published and relaxed
+      // again next round: extra relaxations never change the fixed point.
+      // The pass count is bounded and block-uniform: thread 0 snapshots the
+      // staged count between two barriers (others append to s_nn while a
+      // pass runs), so every thread sees the same bound and every workgroup
+      // reaches the grid barrier.
+      for (uint32_t it = 0, lo = 0; it < p.local_hops; it++) {
+        if (threadIdx.x == 0) s_hi = s_nn < PQ_CAP ? s_nn : PQ_CAP;
+        __syncthreads();
+        const uint32_t hi = s_hi;
+        if (hi <= lo) break;
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) relax(s_near[k]);
+        lo = hi;
+        __syncthreads();  // the pass's appends done, and s_hi read by all
+      }
       publish(s_near, s_nn, &C[0], q[cur ^ 1], 0, &s_base);
       __syncthreads();
       publish(s_far, s_nf, &C[1], far[fcur], nfar, &s_base);
@@ -500,6 +519,11 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   p.round_cap = std::min<uint64_t>(64 * V + 1024, 0x08000000ull);
   p.timeout = ctx.wall_ticks_per_ms() * 20000;  // 20 s per barrier wait
   p.stamp0 = w.stamp;
+  // one pass measured best on the 4096^2 grid (156 -> 143 ms; rounds 12.4K ->
+  // 6.5K): later passes find few new vertices, because the neighbours of a
+  // workgroup's frontier were mostly claimed by other workgroups, and each
+  // pass costs a chain of device-scope atomics
+  p.local_hops = ctx.tuning.sssp_local_hops ? ctx.tuning.sssp_local_hops : 1;
   p.max_deg = (uint32_t)std::max<uint64_t>(g.max_deg, 1);
   int per_cu = 0;
   GRUS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(

@@ -106,7 +106,11 @@ typedef struct grus_tuning {
                              when the frontier's out-edges exceed
                              E / sssp_pull_div; 0 = automatic (push
                              only), 0xFFFFFFFF = push only */
-  uint32_t reserved[5];
+  uint32_t sssp_local_hops; /* SSSP on low-degree graphs (one persistent
+                             launch): extra passes in which a workgroup
+                             relaxes the vertices it has just improved before
+                             the grid barrier; 0 = automatic (1) */
+  uint32_t reserved[4];
 } grus_tuning;
 
 /* ---- autotuning (no reference counterpart: the reference hard-codes its

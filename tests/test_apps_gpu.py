@@ -328,3 +328,21 @@ def test_sssp_pull_rounds_match_push(ctx, oracle):
                     assert st.pull_iterations >= 1
                 if div == NEVER_PULL:
                     assert st.pull_iterations == 0
+
+
+def test_sssp_grid_local_hops_invariant(ctx, oracle):
+    # the persistent kernel's in-workgroup hop passes only add relaxations:
+    # distances equal Dijkstra's for any pass count
+    with grus.Graph.grid(ctx, 96, 160, weights=(1, 1000), wseed=2) as g:
+        ref = oracle.sssp(host_csr(g), 5)
+        t0 = ctx.tuning
+        try:
+            for hops in (1, 8, 64):
+                t = ctx.tuning
+                t.sssp_local_hops = hops
+                ctx.tuning = t
+                d, st = grus.sssp(g, 5)
+                print("hops %d rounds %d" % (hops, st.iterations), flush=True)
+                assert (d == ref).all()
+        finally:
+            ctx.tuning = t0
