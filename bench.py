@@ -138,7 +138,7 @@ def main():
                     help="alpha,beta,cutoff: use these BFS thresholds instead of autotuning "
                          "(e.g. the values an unprofiled run's autotuner printed)")
     ap.add_argument("--only", default="",
-                    help="comma list of extras to run: pagerank,cc,sssp,serving,capped")
+                    help="comma list of extras to run: pagerank,cc,sssp,cu,serving,capped")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device-to-device copy in its own timed window")
     ap.add_argument("--regions-out", default="",
@@ -272,6 +272,8 @@ def run_extra(grus, O, ctx, g, h, threads, args):
         run_sssp(grus, O, ctx, threads, args, out)
     if want("serving"):
         out["serving_rmat_s22"] = run_serving(grus)
+    if want("cu"):
+        out["bfs_cu_scaling"] = run_cu_scaling(grus, ctx, g, args)
     if want("capped"):
         try:
             out["capped_rmat_s27"] = run_capped(grus, ctx)
@@ -347,6 +349,31 @@ def run_sssp(grus, O, ctx, threads, args, out):
             grid["cpu_GTEPS"] = round(st.edges_reached / cms / 1e6, 3)
         out["sssp_grid4096"] = grid
         log("sssp grid: %.2f ms" % ms)
+
+
+def run_cu_scaling(grus, ctx, g, args):
+    """BFS GTEPS on the headline graph with the context restricted to the
+    first 1/8, 1/4, 1/2 and all of the CUs (co-location sizing; the analogue
+    of the reference's SM-limit experiments)."""
+    off = g.csr()[0]
+    deg = np.diff(off.astype(np.int64))
+    srcs = pick_sources(deg, 16, seed=5)
+    total = ctx.cu_count[1]
+    res = {}
+    try:
+        for frac in (8, 4, 2, 1):
+            n = max(1, total // frac)
+            ctx.set_cu_limit(n)
+            for s in srcs[:2]:
+                grus.bfs(g, s, copy=False)
+            er, t0 = 0, time.perf_counter()
+            for s in srcs:
+                er += grus.bfs(g, s, copy=False)[1].edges_reached
+            res["cus_%d" % n] = round(er / (time.perf_counter() - t0) / 1e9, 1)
+    finally:
+        ctx.set_cu_mask(None)
+    log("bfs GTEPS by CU count: %s" % res)
+    return res
 
 
 def run_serving(grus):

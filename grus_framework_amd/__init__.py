@@ -85,6 +85,35 @@ class Context:
     def tuning(self, t):
         capi.check(_lib().grus_set_tuning(self.h, C.byref(t)))
 
+    @property
+    def cu_count(self):
+        """(CUs the context may use, CUs of the device)."""
+        a, t = C.c_int(), C.c_int()
+        capi.check(_lib().grus_cu_count(self.h, C.byref(a), C.byref(t)))
+        return a.value, t.value
+
+    def set_cu_mask(self, words=None):
+        """Run on the CUs whose bits are set (32 per word); None: every CU.
+        Co-location analogue of the reference's tools/sm_limit.cu."""
+        if not words:
+            capi.check(_lib().grus_set_cu_mask(self.h, None, 0))
+            return
+        arr = (C.c_uint32 * len(words))(*[int(w) & 0xFFFFFFFF for w in words])
+        capi.check(_lib().grus_set_cu_mask(self.h, arr, len(words)))
+
+    def set_cu_limit(self, n):
+        """Run on CUs 0..n-1 of the mask numbering (n >= the device's CUs: all)."""
+        total = self.cu_count[1]
+        if n >= total:
+            self.set_cu_mask(None)
+            return
+        if n < 1:
+            raise ValueError("at least one CU")
+        words = [0] * ((total + 31) // 32)
+        for i in range(n):
+            words[i // 32] |= 1 << (i % 32)
+        self.set_cu_mask(words)
+
     def close(self):
         if self.h:
             capi.check(_lib().grus_fini(self.h))

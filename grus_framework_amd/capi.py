@@ -77,6 +77,8 @@ SIGNATURES = [
     ("grus_device_count", _i, [C.POINTER(_i)]),
     ("grus_get_tuning", _i, [_vp, C.POINTER(Tuning)]),
     ("grus_set_tuning", _i, [_vp, C.POINTER(Tuning)]),
+    ("grus_set_cu_mask", _i, [_vp, C.POINTER(C.c_uint32), C.c_uint32]),
+    ("grus_cu_count", _i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     ("grus_gr_header", _i, [C.c_char_p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i)]),
     ("grus_gr_read", _i, [C.c_char_p, _vp, _vp, _vp]),
     ("grus_gr_write", _i, [C.c_char_p, _u64, _u64, _vp, _vp, _vp]),

@@ -94,6 +94,22 @@ int grus_get_tuning(grus_ctx *ctx, grus_tuning *out) {
   });
 }
 
+int grus_set_cu_mask(grus_ctx *ctx, const uint32_t *mask, uint32_t mask_words) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (mask_words) need(mask, "mask");
+    ctx->c->set_cu_mask(mask, mask_words);
+  });
+}
+
+int grus_cu_count(grus_ctx *ctx, int *active, int *total) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (active) *active = ctx->c->cu_count;
+    if (total) *total = ctx->c->cu_total;
+  });
+}
+
 int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in) {
   return guard([&] {
     need(ctx, "ctx");

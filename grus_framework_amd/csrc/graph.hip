@@ -67,7 +67,7 @@ Context::Context(int dev) {
   activate();
   hipDeviceProp_t prop;
   GRUS_HIP(hipGetDeviceProperties(&prop, dev));
-  cu_count = prop.multiProcessorCount;
+  cu_count = cu_total = prop.multiProcessorCount;
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
     wall_khz = khz;
@@ -92,6 +92,26 @@ Context::~Context() {
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::set_cu_mask(const uint32_t *mask, uint32_t words) {
+  activate();
+  int n = 0;
+  for (uint32_t i = 0; i < words; i++)
+    for (int b = 0; b < 32; b++)
+      if ((mask[i] >> b) & 1u && (int)(i * 32 + b) < cu_total) n++;
+  if (words && n == 0) raise(GRUS_E_INVALID, "CU mask selects no compute unit of the device");
+  hipStream_t s = nullptr;
+  if (words) {
+    GRUS_HIP(hipExtStreamCreateWithCUMask(&s, words, mask));
+  } else {
+    GRUS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  sync();
+  GRUS_HIP(hipStreamDestroy(stream));
+  stream = s;
+  cu_masked = words != 0;
+  cu_count = words ? n : cu_total;
 }
 
 void Context::read_ctl(size_t n) {

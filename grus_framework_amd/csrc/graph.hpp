@@ -65,7 +65,9 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int cu_count = 256;
+  int cu_count = 256;  // CUs the context's stream may use (all, or its CU mask)
+  int cu_total = 256;  // CUs of the device
+  bool cu_masked = false;
   int wall_khz = 100000;  // device wall clock rate (wall_clock64 ticks per ms)
   grus_tuning tuning{};
   uint64_t *h_ctl = nullptr;  // pinned host mirror of the control block
@@ -77,6 +79,9 @@ struct Context {
   void activate() const { GRUS_HIP(hipSetDevice(device)); }
   // copy the first n u64 of the device control block to h_ctl and wait
   void read_ctl(size_t n);
+  // replace the stream by one restricted to the CUs set in mask (words == 0:
+  // all CUs); waits for the old stream first
+  void set_cu_mask(const uint32_t *mask, uint32_t words);
   uint64_t wall_ticks_per_ms() const { return (uint64_t)(wall_khz > 0 ? wall_khz : 100000); }
 };
 

@@ -570,7 +570,9 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   uint64_t launches = 0, rounds = 0, relaxed_vertices = 0, pull_rounds = 0;
   // low-degree graphs (grids, road-like): one persistent cooperative launch
   static const bool no_persist = getenv("GRUS_NO_PERSISTENT") != nullptr;
-  const bool persistent = g.max_deg <= 32 && !no_persist;
+  // (not under a CU mask: a cooperative grid must be co-resident, and the
+  // runtime's residency check does not know the mask)
+  const bool persistent = g.max_deg <= 32 && !no_persist && !ctx.cu_masked;
   GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   sssp::k_init<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(dist, V, src,
                                                                  w.q[0].as<uint32_t>());

@@ -138,6 +138,16 @@ int grus_fini(grus_ctx *ctx);
 int grus_device_count(int *count);
 int grus_get_tuning(grus_ctx *ctx, grus_tuning *out);
 int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in);
+/* Restrict the context's work to a set of compute units: bit i of mask
+ * (32 CUs per word) enables CU i, as hipExtStreamCreateWithCUMask numbers
+ * them; mask_words = 0 restores every CU.  Replaces the reference's
+ * co-location tool, which parks spinning kernels on the SMs to take them
+ * away (tools/sm_limit.cu:5-25, used with MPS, tools/mps_start.sh:6-7).
+ * Results never depend on the mask.  Servers created afterwards run on their
+ * own streams and are not restricted. */
+int grus_set_cu_mask(grus_ctx *ctx, const uint32_t *mask, uint32_t mask_words);
+/* CUs the context may use and the device's total. */
+int grus_cu_count(grus_ctx *ctx, int *active, int *total);
 
 /* ---- host-side .gr I/O (no device needed) ----
  * Galois .gr v1: u64 header {1, sizeEdgeTy, N, M}, u64[N] end offsets,

@@ -70,3 +70,29 @@ def test_autotune_keeps_results_and_caches(ctx, tmp_path):
             assert again.bfs_alpha == tb.bfs_alpha and again.sssp_delta == ts.sssp_delta
         finally:
             ctx.tuning = t0
+
+
+def test_cu_mask_keeps_results(oracle):
+    # a context restricted to a quarter of the CUs computes the same levels,
+    # distances, labels and ranks; grids take the host-driven SSSP path
+    # under a mask (a cooperative grid must be co-resident)
+    with grus.Context(0) as c:
+        active, total = c.cu_count
+        assert active == total > 0
+        with grus.Graph.rmat(c, 14, weights=(1, 255), wseed=3) as g, \
+                grus.Graph.grid(c, 64, 80, weights=(1, 1000), wseed=2) as gg:
+            s = g.info.max_degree_vertex
+            full = [grus.bfs(g, s)[0], grus.sssp(g, s)[0], grus.cc(g)[0], grus.pagerank(g)[0],
+                    grus.sssp(gg, 0)[0]]
+            c.set_cu_limit(max(1, total // 4))
+            assert c.cu_count == (max(1, total // 4), total)
+            part = [grus.bfs(g, s)[0], grus.sssp(g, s)[0], grus.cc(g)[0], grus.pagerank(g)[0],
+                    grus.sssp(gg, 0)[0]]
+            for a, b in zip(full, part):
+                assert (a.view(np.uint32) == b.view(np.uint32)).all()
+            assert (part[4] == oracle.sssp(oracle.CSR(*gg.csr()), 0)).all()
+            with pytest.raises(grus.GrusError) as e:
+                c.set_cu_mask([0])
+            assert e.value.code == grus.capi.E_INVALID
+            c.set_cu_mask(None)
+            assert c.cu_count == (total, total)
