@@ -104,10 +104,10 @@ __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uin
 constexpr int PULL_UNROLL = 8;
 constexpr int PULL_BATCH = 4;  // in-edges tested per dependent load round
 __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_off,
-                                                const uint64_t *front, uint64_t *visited,
-                                                uint64_t *front_next, uint32_t *label,
-                                                uint32_t next_level, uint32_t K, uint32_t *longq,
-                                                StepCounters *cnt, uint64_t W) {
+                                                const uint32_t *first_col, const uint64_t *front,
+                                                uint64_t *visited, uint64_t *front_next,
+                                                uint32_t *label, uint32_t next_level, uint32_t K,
+                                                uint32_t *longq, StepCounters *cnt, uint64_t W) {
   uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
   uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
   uint32_t lane = lane_id();
@@ -131,9 +131,13 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
         end[j] = in.off[v + 1];
       }
     }
+    // the first neighbour: from the resident copy when the rows may live in
+    // host memory (capped placement), else from the column array
 #pragma unroll
-    for (int j = 0; j < PULL_UNROLL; j++)
-      first[j] = (beg[j] < end[j]) ? edge_col(in, beg[j]) : 0u;
+    for (int j = 0; j < PULL_UNROLL; j++) {
+      const uint32_t v = (uint32_t)((w0 + j) * 64 + lane);
+      first[j] = (beg[j] < end[j]) ? (first_col ? first_col[v] : edge_col(in, beg[j])) : 0u;
+    }
 #pragma unroll
     for (int j = 0; j < PULL_UNROLL; j++)
       found[j] = (beg[j] < end[j]) && bm_test(front, first[j]);
@@ -285,8 +289,9 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
         launches += 2;
       }
       bfs::k_pull<<<grid_cap(ctx, div_up(W, bfs::PULL_UNROLL) * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          in, out.off, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),
-          w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K, w.longq.as<uint32_t>(), cnt, W);
+          in, out.off, g.first_col.as<uint32_t>(), w.front[cur].as<uint64_t>(),
+          w.visited.as<uint64_t>(), w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K,
+          w.longq.as<uint32_t>(), cnt, W);
       GRUS_LAUNCH_CHECK();
       bfs::k_pull_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
           in, out.off, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),

@@ -137,12 +137,19 @@ CsrView Graph::csr() const {
 
 uint64_t Graph::device_bytes() const {
   uint64_t b = 0;
-  for (const Buf *x : {&off, &col, &w, &in_off, &in_col, &in_w, &unreachable_bm, &result})
+  for (const Buf *x :
+       {&off, &col, &w, &in_off, &in_col, &in_w, &unreachable_bm, &result, &first_col})
     if (x->kind == Mem::Device) b += x->bytes;
   return b;
 }
 
 namespace {
+
+__global__ void k_first_col(const uint64_t *off, const uint32_t *col, uint64_t V, uint32_t *out) {
+  uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; v < V; v += stride) out[v] = off[v] < off[v + 1] ? col[off[v]] : 0u;
+}
 
 // ---- derived quantities ----
 __global__ void k_degree_stats(const uint64_t *off, uint64_t V, unsigned long long *best,
@@ -566,8 +573,15 @@ void graph_apply_placement(Graph &g, const grus_place *place) {
   if (!g.symmetric) raise(GRUS_E_UNSUPPORTED, "capped placement needs a symmetric graph");
   uint64_t cap = place->hbm_cap_bytes;
   uint64_t per_edge = g.weighted ? 8 : 4;
-  uint64_t fixed = (g.V + 1) * 8;
-  if (cap && cap < fixed) raise(GRUS_E_BUDGET, "HBM cap below the offsets array size");
+  // always resident: the offsets and the first neighbour of every row
+  uint64_t fixed = (g.V + 1) * 8 + g.V * 4;
+  if (cap && cap < fixed)
+    raise(GRUS_E_BUDGET, "HBM cap below the offsets and first-neighbour arrays (12 bytes per vertex)");
+  g.first_col.alloc(std::max<uint64_t>(g.V, 1) * 4);
+  k_first_col<<<grid_for(ctx, g.V), BLOCK, 0, ctx->stream>>>(
+      g.off.as<uint64_t>(), g.col.as<uint32_t>(), g.V, g.first_col.as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  ctx->sync();
   uint64_t fit = cap ? (cap - fixed) / per_edge : g.E;
   uint64_t split = std::min<uint64_t>(fit, g.E);
   // a multiple of 64 edges, so that vector column loads never straddle the

@@ -35,7 +35,8 @@ def check_same(got, ref):
 def test_capped_matches_hbm(ctx, reference, frac):
     src, (off, col, w), ref = reference
     V, E = len(off) - 1, len(col)
-    cap = (V + 1) * 8 + int(frac * E) * 8  # offsets + frac of (col, weight) pairs
+    # offsets + first neighbour per vertex + frac of the (col, weight) pairs
+    cap = (V + 1) * 8 + V * 4 + int(frac * E) * 8
     pl = grus.place("capped", hbm_cap_bytes=cap)
     with grus.Graph.from_csr(ctx, off, col, w, symmetric=True, placement=pl) as g:
         assert g.info.placement == grus.PLACE_CAPPED
