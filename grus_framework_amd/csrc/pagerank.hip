@@ -13,7 +13,7 @@
 // graphs).  Dangling mass is dropped, as in the reference.
 //
 // Kernel (SpMV over the in-edge CSR): the edge array is cut into windows of
-// PR_WIN edges, one per workgroup, PR_EPT consecutive edges per thread.
+// WIN edges, one per workgroup, WIN_EPT consecutive edges per thread.
 // Each thread gathers contrib[col[e]] for its edges into registers and the
 // window is reduced by row segments (merge-style): rows that start and end
 // inside the window are finished there, the segment of the row that entered
@@ -43,11 +43,10 @@
 #include "advance.hpp"
 #include "apps.hpp"
 #include "prims.hpp"
+#include "window.hpp"
 
 namespace mgg {
 
-constexpr int PR_EPT = 8;                // edges per thread
-constexpr int PR_WIN = PR_EPT * BLOCK;    // edges per window
 constexpr int PB_BITS = 14;   // blocked layout: 16K destinations per bin
 
 struct PrWork {
@@ -67,19 +66,24 @@ struct PrWork {
   const void *pb_for = nullptr;  // offsets the layout was built on
 };
 
-namespace pr {
-
-// win_row[k] = first row r with off[r] >= k*PR_WIN (V if none), k in [0, nwin]
 __global__ void k_window_rows(const uint64_t *off, uint64_t V, uint64_t nwin, uint32_t *win_row) {
   uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; r <= V; r += stride) {
     // rows are owned by the window of their start offset; row V is a sentinel
-    int64_t kprev = (r == 0) ? -1 : (int64_t)(off[r - 1] / PR_WIN);
-    int64_t kcur = (r == V) ? (int64_t)nwin : (int64_t)(off[r] / PR_WIN);
+    int64_t kprev = (r == 0) ? -1 : (int64_t)(off[r - 1] / WIN);
+    int64_t kcur = (r == V) ? (int64_t)nwin : (int64_t)(off[r] / WIN);
     for (int64_t k = kprev + 1; k <= kcur; k++) win_row[k] = (uint32_t)r;
   }
 }
+
+void build_window_rows(Context &ctx, const uint64_t *off, uint64_t V, uint64_t nwin,
+                       uint32_t *win_row) {
+  k_window_rows<<<grid_cap(ctx, V + 1, BLOCK), BLOCK, 0, ctx.stream>>>(off, V, nwin, win_row);
+  GRUS_LAUNCH_CHECK();
+}
+
+namespace pr {
 
 // key = (degree descending, id ascending) for the relabelling sort
 __global__ void k_degree_keys(const uint64_t *off, uint64_t V, uint64_t *keys) {
@@ -244,237 +248,46 @@ __device__ __forceinline__ void finish(uint32_t r, float s, const uint64_t *out_
   }
 }
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// One window of PR_WIN in-edges per workgroup, PR_EPT consecutive edges per
-// thread, reduced by row segments (merge-style CSR SpMV):
-//   1. each thread gathers the contributions of its PR_EPT edges into
-//      registers (two 16-byte column loads, non-temporal);
-//   2. the window's segments are staged in LDS: segment 0 is the head (the
-//      part of a row that began in an earlier window, possibly empty), then
-//      every non-empty row owned by the window in order; empty owned rows
-//      are finished on the spot with a zero sum;
-//   3. each thread finds the segment of its first edge by binary search and
-//      walks its edges: segments that begin and end inside its range are
-//      finished directly, the one it enters keeps its partial until the
-//      carry is known, the one it leaves open is its scan value (and is
-//      finished by the next thread when it ends exactly at that boundary);
-//   4. a segmented inclusive scan over threads (keys = open segment, 64-lane
-//      shuffles then the 4 waves through LDS) gives every thread the sum of
-//      the open partials of the threads before it in the same segment.
-// Sums are sequential within a thread and a fixed tree across threads, so
-// results are bit-identical run to run.  The head partial goes to
-// head_sum[k], the partial of a row that continues past the window to
-// tail_sum[k]; k_fixup completes those rows.
+// PageRank as a window reduction: sum of contrib[u] over the in-edges
 template <bool kLast>
-__global__ __launch_bounds__(BLOCK) void k_pull_window(CsrView in, const uint64_t *out_off,
-                                                       const uint32_t *win_row,
-                                                       const float *contrib, float *next,
-                                                       float *head_sum, float *tail_sum,
-                                                       float base, float alpha, uint64_t nwin,
+struct PrOp {
+  using T = float;
+  const float *contrib;
+  float *next;
+  const uint64_t *out_off;
+  float base, alpha;
+  float *head_sum, *tail_sum;
+  __device__ T identity() const { return 0.0f; }
+  __device__ T combine(T a, T b) const { return a + b; }
+  __device__ T value(uint64_t, uint32_t c) const { return contrib[c]; }
+  __device__ void finish(uint32_t r, T s) const {
+    pr::finish<kLast>(r, s, out_off, next, base, alpha);
+  }
+  __device__ void empty(uint32_t r) const { finish(r, 0.0f); }
+  __device__ void head(uint64_t k, T s) const { head_sum[k] = s; }
+  __device__ void tail(uint64_t k, T s) const { tail_sum[k] = s; }
+  __device__ T head_at(uint64_t k) const { return head_sum[k]; }
+  __device__ T tail_at(uint64_t k) const { return tail_sum[k]; }
+};
+
+template <bool kLast>
+__global__ __launch_bounds__(BLOCK) void k_pull_window(CsrView in, const uint32_t *win_row,
+                                                       PrOp<kLast> op, uint64_t nwin,
                                                        bool xcd_order) {
-  constexpr int NW = BLOCK / WAVE;
-  constexpr uint32_t NONE = 0xFFFFFFFFu;
-  __shared__ uint32_t s_start[PR_WIN + 2];  // segment starts relative to e0
-  __shared__ uint32_t s_row[PR_WIN + 2];    // row of segment j >= 1
-  __shared__ uint32_t s_cnt[NW];
-  __shared__ float s_wsum[NW];
-  __shared__ uint32_t s_wkey[NW];
+  __shared__ WindowShared sh;
   // blocked layout: consecutive windows gather from the same bin, so they
   // are kept on one XCD (its L2); the plain pull gathers from anywhere and
   // runs slightly faster in dispatch order
   const uint64_t k = xcd_order ? xcd_block() : blockIdx.x;
   if (k >= nwin) return;
-  const uint64_t e0 = k * PR_WIN;
-  const uint64_t e1 = e0 + PR_WIN < in.E ? e0 + PR_WIN : in.E;
-  const uint32_t len = (uint32_t)(e1 - e0);
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
-  const uint32_t p0 = tid * PR_EPT;
-
-  // 1. gather
-  float v[PR_EPT];
-  {
-    const uint64_t g0 = e0 + p0;
-    // a thread's PR_EPT columns lie wholly on one side of the HBM / host
-    // split (a multiple of 64) and are 32-byte aligned on either side
-    if (p0 + PR_EPT <= len) {
-      const u32x4 *cp = reinterpret_cast<const u32x4 *>(
-          g0 < in.split ? in.col + g0 : in.col_far + (g0 - in.split));
-      const u32x4 c0 = __builtin_nontemporal_load(cp);
-      const u32x4 c1 = __builtin_nontemporal_load(cp + 1);
-      v[0] = contrib[c0.x];
-      v[1] = contrib[c0.y];
-      v[2] = contrib[c0.z];
-      v[3] = contrib[c0.w];
-      v[4] = contrib[c1.x];
-      v[5] = contrib[c1.y];
-      v[6] = contrib[c1.z];
-      v[7] = contrib[c1.w];
-    } else {
-#pragma unroll
-      for (int q = 0; q < PR_EPT; q++) {
-        v[q] = 0.0f;
-        if (p0 + q < len) v[q] = contrib[edge_col(in, g0 + q)];
-      }
-    }
-  }
-
-  // 2. stage the segments (order-preserving compaction of non-empty rows)
-  const uint32_t r_begin = win_row[k], r_end = win_row[k + 1];
-  uint32_t n = 0;  // non-empty owned rows so far
-  for (uint32_t rb = r_begin; rb < r_end; rb += BLOCK) {
-    const uint32_t r = rb + tid;
-    uint64_t a = 0, b = 0;
-    if (r < r_end) {
-      a = in.off[r];
-      b = in.off[r + 1];
-      if (b == a) finish<kLast>(r, 0.0f, out_off, next, base, alpha);
-    }
-    const bool ne = r < r_end && b > a;
-    const uint64_t m = __ballot(ne);
-    if (lane == 0) s_cnt[wid] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      before += (w < (int)wid) ? s_cnt[w] : 0u;
-      total += s_cnt[w];
-    }
-    if (ne) {
-      const uint32_t j = 1 + n + before + lane_rank(m);
-      s_start[j] = (uint32_t)(a - e0);
-      s_row[j] = r;
-    }
-    n += total;
-    __syncthreads();  // s_cnt is reused by the next chunk
-  }
-  if (tid == 0) {
-    s_start[0] = 0;
-    s_start[n + 1] = len;
-  }
-  __syncthreads();
-
-  // 3. walk
-  const bool active = p0 < len;
-  uint32_t key = NONE;        // open segment at the end of this thread's range
-  float x = 0.0f;             // its partial
-  uint32_t first_j = NONE;    // segment entered at p0, if this thread closes it
-  float first_sum = 0.0f;
-  uint32_t j_start = NONE;    // segment of edge p0
-  if (active) {
-    uint32_t lo = 0, hi = n + 1;  // s_start[lo] <= p0 < s_start[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_start[mid] <= p0) lo = mid;
-      else hi = mid;
-    }
-    uint32_t j = lo;
-    j_start = lo;
-    uint32_t end = s_start[j + 1];
-    bool first = true;
-    float sum = 0.0f;
-#pragma unroll
-    for (int q = 0; q < PR_EPT; q++) {
-      const uint32_t p = p0 + q;
-      if (p < len) {
-        while (p >= end) {  // segment j closes before edge p
-          if (first) {
-            first_j = j;
-            first_sum = sum;
-            first = false;
-          } else if (j == 0) {
-            head_sum[k] = sum;
-          } else {
-            finish<kLast>(s_row[j], sum, out_off, next, base, alpha);
-          }
-          sum = 0.0f;
-          j++;
-          end = s_start[j + 1];
-        }
-        sum += v[q];
-      }
-    }
-    key = j;
-    x = sum;
-    if (first) first_j = NONE;  // never closed a segment: all of it is open
-  }
-
-  // 4. segmented inclusive scan of the open partials
-  float S = x;
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    const float y = __shfl_up(S, off);
-    const uint32_t ky = __shfl_up(key, off);
-    if (lane >= (uint32_t)off && ky == key) S += y;
-  }
-  if (lane == WAVE - 1) {
-    s_wsum[wid] = S;
-    s_wkey[wid] = key;
-  }
-  __syncthreads();
-  // carry into this wave: total of segment s_wkey[wid-1] through wave wid-1
-  float cw = 0.0f;
-  uint32_t kw = NONE;
-  if (wid > 0) {
-    kw = s_wkey[wid - 1];
-    cw = s_wsum[wid - 1];
-    for (int w = (int)wid - 2; w >= 0 && s_wkey[w] == kw; w--) cw += s_wsum[w];
-  }
-  if (key == kw && kw != NONE) S += cw;
-  float prevS = __shfl_up(S, 1);
-  uint32_t prevK = __shfl_up(key, 1);
-  if (lane == 0) {
-    prevS = cw;
-    prevK = kw;
-  }
-  if (!active) return;
-  // the previous thread's open segment ended exactly at p0: finish it here
-  if (prevK != NONE && prevK != j_start) {
-    if (prevK == 0) head_sum[k] = prevS;
-    else finish<kLast>(s_row[prevK], prevS, out_off, next, base, alpha);
-  }
-  if (first_j != NONE) {
-    const float total = (prevK == first_j ? prevS : 0.0f) + first_sum;
-    if (first_j == 0) head_sum[k] = total;
-    else finish<kLast>(s_row[first_j], total, out_off, next, base, alpha);
-  }
-  // the window's last segment is closed (or left open) by the thread of the
-  // last edge
-  if (p0 < len && len - p0 <= PR_EPT) {
-    if (key == 0) {
-      head_sum[k] = S;
-    } else {
-      const uint32_t r = s_row[key];
-      if (in.off[r + 1] > e1) tail_sum[k] = S;
-      else finish<kLast>(r, S, out_off, next, base, alpha);
-    }
-  }
+  window_reduce(in, win_row, k, op, sh);
 }
 
-// rows that span windows: owner window k has a tail partial; add the head
-// partials of the following windows the row reaches, in order
 template <bool kLast>
-__global__ void k_fixup(CsrView in, const uint64_t *out_off, const uint32_t *win_row,
-                        const float *head_sum, const float *tail_sum, float *next, float base,
-                        float alpha, uint64_t nwin) {
+__global__ void k_fixup(CsrView in, const uint32_t *win_row, PrOp<kLast> op, uint64_t nwin) {
   uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (k >= nwin) return;
-  uint32_t r_begin = win_row[k], r_end = win_row[k + 1];
-  if (r_end == r_begin) return;  // owns no row
-  uint32_t r = r_end - 1;        // last owned row
-  uint64_t b = in.off[r + 1];
-  uint64_t e1 = (k + 1) * PR_WIN < in.E ? (k + 1) * PR_WIN : in.E;
-  if (b <= e1) return;  // completed inside its window
-  float s = tail_sum[k];
-  uint64_t klast = (b - 1) / PR_WIN;
-  for (uint64_t j = k + 1; j <= klast; j++) s += head_sum[j];
-  float rank = finish_row(s, base, alpha);
-  if (kLast) {
-    next[r] = rank;
-  } else {
-    uint64_t d = out_off[r + 1] - out_off[r];
-    next[r] = d ? rank / (float)d : 0.0f;
-  }
+  window_fixup(in, win_row, k, op);
 }
 
 }  // namespace pr
@@ -641,15 +454,13 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
     in = out;
   }
   if (w.built_for != in.off || w.cap_v < V) {
-    w.nwin = in.E / PR_WIN + 1;
+    w.nwin = in.E / WIN + 1;
     w.win_row.alloc((w.nwin + 1) * 4);
     w.head_sum.alloc(w.nwin * 4);
     w.tail_sum.alloc(w.nwin * 4);
     w.contrib[0].alloc(V * 4);
     w.contrib[1].alloc(V * 4);
-    pr::k_window_rows<<<grid_cap(ctx, V + 1, BLOCK), BLOCK, 0, ctx.stream>>>(in.off, V, w.nwin,
-                                                                           w.win_row.as<uint32_t>());
-    GRUS_LAUNCH_CHECK();
+    build_window_rows(ctx, in.off, V, w.nwin, w.win_row.as<uint32_t>());
     w.built_for = in.off;
     w.cap_v = V;
   }
@@ -683,21 +494,21 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
       launches++;
     }
     if (last) {
-      pr::k_pull_window<true><<<nwin, BLOCK, 0, ctx.stream>>>(
-          in, out.off, w.win_row.as<uint32_t>(), c, next, w.head_sum.as<float>(),
-          w.tail_sum.as<float>(), base, alpha, w.nwin, blocked);
+      const pr::PrOp<true> op{c, next, out.off, base, alpha, w.head_sum.as<float>(),
+                              w.tail_sum.as<float>()};
+      pr::k_pull_window<true><<<nwin, BLOCK, 0, ctx.stream>>>(in, w.win_row.as<uint32_t>(), op,
+                                                             w.nwin, blocked);
       GRUS_LAUNCH_CHECK();
-      pr::k_fixup<true><<<fix_grid, BLOCK, 0, ctx.stream>>>(
-          in, out.off, w.win_row.as<uint32_t>(), w.head_sum.as<float>(), w.tail_sum.as<float>(),
-          next, base, alpha, w.nwin);
+      pr::k_fixup<true><<<fix_grid, BLOCK, 0, ctx.stream>>>(in, w.win_row.as<uint32_t>(), op,
+                                                           w.nwin);
     } else {
-      pr::k_pull_window<false><<<nwin, BLOCK, 0, ctx.stream>>>(
-          in, out.off, w.win_row.as<uint32_t>(), c, next, w.head_sum.as<float>(),
-          w.tail_sum.as<float>(), base, alpha, w.nwin, blocked);
+      const pr::PrOp<false> op{c, next, out.off, base, alpha, w.head_sum.as<float>(),
+                               w.tail_sum.as<float>()};
+      pr::k_pull_window<false><<<nwin, BLOCK, 0, ctx.stream>>>(in, w.win_row.as<uint32_t>(), op,
+                                                              w.nwin, blocked);
       GRUS_LAUNCH_CHECK();
-      pr::k_fixup<false><<<fix_grid, BLOCK, 0, ctx.stream>>>(
-          in, out.off, w.win_row.as<uint32_t>(), w.head_sum.as<float>(), w.tail_sum.as<float>(),
-          next, base, alpha, w.nwin);
+      pr::k_fixup<false><<<fix_grid, BLOCK, 0, ctx.stream>>>(in, w.win_row.as<uint32_t>(), op,
+                                                            w.nwin);
     }
     GRUS_LAUNCH_CHECK();
     launches += 2;

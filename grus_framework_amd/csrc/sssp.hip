@@ -19,12 +19,16 @@
 #include "advance.hpp"
 #include "apps.hpp"
 #include "persist.hpp"
+#include "window.hpp"
 
 namespace mgg {
 
 struct SsspWork {
   Buf q[2], far[2], near_stamp, far_stamp, pctl;
-  Buf front_bm, longq;  // pull rounds: frontier bitmap, rows handed to waves
+  Buf front_bm;               // pull rounds: frontier bitmap
+  Buf win_row, head_min, tail_min;  // pull rounds: window table of the in-edges
+  const void *win_for = nullptr;    // in-edge offsets the table was built on
+  uint64_t nwin = 0;
   PushWork push;
   uint64_t cap_v = 0;
   uint32_t stamp = 0;  // last stamp handed out; stamps only grow
@@ -38,7 +42,7 @@ struct SsspWork {
     near_stamp.alloc((V + 1) * 4);
     far_stamp.alloc((V + 1) * 4);
     front_bm.alloc((V / 64 + 2) * 8);
-    longq.alloc((V + 1) * 4);
+    win_for = nullptr;
     fill_u32(ctx, near_stamp.as<uint32_t>(), V + 1, 0);
     fill_u32(ctx, far_stamp.as<uint32_t>(), V + 1, 0);
     push.ensure(V);
@@ -100,113 +104,100 @@ struct job_t {
 };
 
 // ---- pull rounds (large frontiers) ----
-// A vertex whose distance is below T_lo (the threshold of the previous
-// epoch) is final and skipped.  Every other vertex takes the minimum of
-// dist[u] + w over its in-neighbours u in the frontier; only its own thread
-// writes dist[v], so no atomics are needed on distances.  Reads of dist[u]
-// may see values lowered in this same round: still lengths of real paths.
-constexpr uint32_t PULL_K = 32;  // rows longer than this go to a wave
+// A pull round is a window reduction (window.hpp) over the in-edges: the
+// value of edge (u, v) is dist[u] + w when u is in the frontier bitmap, the
+// reduction is the minimum, and finishing row v lowers dist[v] when the
+// minimum is smaller.  Only row v's finisher writes dist[v], so no atomics
+// are needed on distances; reads of dist[u] may see values lowered in this
+// same round, which are still lengths of real paths.  Improved vertices join
+// the near queue (below T) or the far list (once per epoch, by stamp),
+// staged in LDS by the window kernel and appended directly by the fixup.
+struct PullMinOp {
+  using T = uint32_t;
+  CsrView in;
+  const uint64_t *front;
+  uint32_t *dist;
+  const uint64_t *out_off;
+  uint32_t T_thr;
+  uint32_t *near_stamp, near_id, *far_stamp, far_id;
+  uint32_t *head_min, *tail_min;
+  uint32_t *qn, *qfar;
+  StepCounters *cnt;
+  // LDS staging (window kernel); null in the fixup
+  uint32_t *s_near, *s_far, *s_nn, *s_nf;
+  unsigned long long *s_edges;
 
-// classify an improved vertex: near queue below T, else the far list once
-// per epoch (its owner is the only writer of both stamps this round)
-__device__ __forceinline__ void pull_classify(uint32_t v, uint32_t best, uint32_t T,
-                                              uint32_t *near_stamp, uint32_t near_id,
-                                              uint32_t *far_stamp, uint32_t far_id, bool *near,
-                                              bool *far) {
-  if (best < T) {
-    *near = true;
-    near_stamp[v] = near_id;
-  } else if (far_stamp[v] != far_id) {
-    far_stamp[v] = far_id;
-    *far = true;
+  __device__ T identity() const { return 0xFFFFFFFFu; }
+  __device__ T combine(T a, T b) const { return a < b ? a : b; }
+  __device__ T value(uint64_t e, uint32_t u) const {
+    if (!bm_test(front, u)) return 0xFFFFFFFFu;
+    const uint64_t c = (uint64_t)dist[u] + edge_w(in, e);
+    return c < INFINIT ? (uint32_t)c : 0xFFFFFFFFu;
   }
+  __device__ void finish(uint32_t r, T best) const {
+    if (best >= dist[r]) return;
+    dist[r] = best;
+    if (best < T_thr) {
+      near_stamp[r] = near_id;
+      const unsigned long long deg = out_off[r + 1] - out_off[r];
+      if (s_near) {
+        s_near[atomicAdd(s_nn, 1u)] = r;
+        atomicAdd(s_edges, deg);
+      } else {
+        qn[atomicAdd(&cnt->next_count, 1u)] = r;
+        atomicAdd((unsigned long long *)&cnt->next_edges, deg);
+      }
+    } else if (far_stamp[r] != far_id) {
+      far_stamp[r] = far_id;
+      if (s_far) s_far[atomicAdd(s_nf, 1u)] = r;
+      else qfar[atomicAdd(&cnt->far_count, 1u)] = r;
+    }
+  }
+  __device__ void empty(uint32_t) const {}
+  __device__ void head(uint64_t k, T t) const { head_min[k] = t; }
+  __device__ void tail(uint64_t k, T t) const { tail_min[k] = t; }
+  __device__ T head_at(uint64_t k) const { return head_min[k]; }
+  __device__ T tail_at(uint64_t k) const { return tail_min[k]; }
+};
+
+__global__ __launch_bounds__(BLOCK) void k_pull_min_window(CsrView in, const uint32_t *win_row,
+                                                           PullMinOp op, uint64_t nwin) {
+  __shared__ WindowShared sh;
+  // a window finishes at most one row per edge (plus none for empty rows)
+  __shared__ uint32_t s_near[WIN + 2], s_far[WIN + 2];
+  __shared__ uint32_t s_nn, s_nf, s_bn, s_bf;
+  __shared__ unsigned long long s_edges;
+  const uint64_t k = blockIdx.x;
+  if (k >= nwin) return;
+  if (threadIdx.x == 0) {
+    s_nn = s_nf = 0;
+    s_edges = 0;
+  }
+  __syncthreads();
+  PullMinOp o = op;
+  o.s_near = s_near;
+  o.s_far = s_far;
+  o.s_nn = &s_nn;
+  o.s_nf = &s_nf;
+  o.s_edges = &s_edges;
+  window_reduce(in, win_row, k, o, sh);
+  __syncthreads();
+  // one global append per list per workgroup
+  if (threadIdx.x == 0) {
+    s_bn = s_nn ? atomicAdd(&op.cnt->next_count, s_nn) : 0u;
+    s_bf = s_nf ? atomicAdd(&op.cnt->far_count, s_nf) : 0u;
+    if (s_edges) atomicAdd((unsigned long long *)&op.cnt->next_edges, s_edges);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < s_nn; i += BLOCK) op.qn[s_bn + i] = s_near[i];
+  for (uint32_t i = threadIdx.x; i < s_nf; i += BLOCK) op.qfar[s_bf + i] = s_far[i];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_pull_relax(CsrView in, const uint64_t *front,
-                                                      uint32_t *dist, uint32_t T_lo, uint32_t T,
-                                                      uint32_t *near_stamp, uint32_t near_id,
-                                                      uint32_t *far_stamp, uint32_t far_id,
-                                                      uint32_t *qn, uint32_t *qfar,
-                                                      uint32_t *longq, StepCounters *cnt,
-                                                      const uint64_t *out_off) {
-  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
-  const uint64_t V = in.V;
-  uint64_t my_edges = 0;
-  // block-uniform loop: block_append needs every thread of the workgroup
-  for (uint64_t b0 = blockIdx.x * (uint64_t)BLOCK; b0 < V; b0 += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t v = b0 + threadIdx.x;
-    bool is_long = false, near = false, far = false;
-    if (v < V) {
-      const uint32_t d = dist[v];
-      const uint64_t a = in.off[v], b = in.off[v + 1];
-      if (d >= T_lo && b > a) {
-        if (b - a > PULL_K) {
-          is_long = true;
-        } else {
-          uint32_t best = d;
-          for (uint64_t e = a; e < b; e++) {
-            const uint32_t u = edge_col(in, e);
-            if (bm_test(front, u)) {
-              const uint64_t c = (uint64_t)dist[u] + edge_w(in, e);
-              if (c < best) best = (uint32_t)c;
-            }
-          }
-          if (best < d) {
-            dist[v] = best;
-            pull_classify((uint32_t)v, best, T, near_stamp, near_id, far_stamp, far_id, &near,
-                          &far);
-          }
-        }
-      }
-    }
-    const uint32_t sn = block_append(&cnt->next_count, near, s_app);
-    if (near) {
-      qn[sn] = (uint32_t)v;
-      my_edges += out_off[v + 1] - out_off[v];
-    }
-    const uint32_t sf = block_append(&cnt->far_count, far, s_app);
-    if (far) qfar[sf] = (uint32_t)v;
-    const uint32_t sl = block_append(&cnt->long_count, is_long, s_app);
-    if (is_long) longq[sl] = (uint32_t)v;
-  }
-  my_edges = wave_sum(my_edges);
-  if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
-}
-
-// rows longer than PULL_K: one wave per vertex, lanes stride its in-edges
-__global__ __launch_bounds__(BLOCK) void k_pull_relax_long(
-    CsrView in, const uint64_t *front, uint32_t *dist, uint32_t T, uint32_t *near_stamp,
-    uint32_t near_id, uint32_t *far_stamp, uint32_t far_id, uint32_t *qn, uint32_t *qfar,
-    const uint32_t *longq, StepCounters *cnt, const uint64_t *out_off) {
-  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
-  const uint32_t lane = lane_id();
-  const uint32_t n = cnt->long_count;
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    const uint32_t v = longq[i];
-    const uint32_t d = dist[v];
-    const uint64_t a = in.off[v], b = in.off[v + 1];
-    uint32_t best = d;
-    for (uint64_t e = a + lane; e < b; e += WAVE) {
-      const uint32_t u = edge_col(in, e);
-      if (bm_test(front, u)) {
-        const uint64_t c = (uint64_t)dist[u] + edge_w(in, e);
-        if (c < best) best = (uint32_t)c;
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor(best, o));
-    if (lane == 0 && best < d) {
-      dist[v] = best;
-      bool near = false, far = false;
-      pull_classify(v, best, T, near_stamp, near_id, far_stamp, far_id, &near, &far);
-      if (near) {
-        qn[atomicAdd(&cnt->next_count, 1u)] = v;
-        atomicAdd((unsigned long long *)&cnt->next_edges, out_off[v + 1] - out_off[v]);
-      }
-      if (far) qfar[atomicAdd(&cnt->far_count, 1u)] = v;
-    }
-  }
+__global__ void k_pull_min_fixup(CsrView in, const uint32_t *win_row, PullMinOp op,
+                                 uint64_t nwin) {
+  uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (k >= nwin) return;
+  window_fixup(in, win_row, k, op);
 }
 
 __global__ void k_init(uint32_t *dist, uint64_t V, uint32_t src, uint32_t *q0) {
@@ -585,16 +576,12 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   int cur = 0, fcur = 0;
   uint64_t nn = persistent ? 0 : 1, mf = src_deg, nfar = 0;
   uint64_t T = delta;
-  uint64_t T_lo = 0;  // distances below it are final (threshold of the previous epoch)
   uint32_t far_id = w.next_stamp(ctx, V);
   const uint64_t round_cap = 64 * V + 1024;
-  // a round pulls when its frontier's out-edges exceed E / pull_div.  The
-  // automatic setting never pulls: on RMAT s24 a pull round (every unsettled
-  // vertex scans all its in-edges, no early exit) costs ~10 ms, more than
-  // the atomic-filtered push of the same frontier
-  const uint32_t pull_div = ctx.tuning.sssp_pull_div;
-  const uint64_t pull_edges =
-      (pull_div == 0 || pull_div == 0xFFFFFFFFu) ? ~0ull : g.E / pull_div;
+  // a round pulls when its frontier's out-edges exceed E / pull_div
+  // (automatic: 16, measured best of 4..128 on RMAT s24; 0xFFFFFFFF: never)
+  const uint32_t pull_div = ctx.tuning.sssp_pull_div ? ctx.tuning.sssp_pull_div : 16;
+  const uint64_t pull_edges = pull_div == 0xFFFFFFFFu ? ~0ull : g.E / pull_div;
   const bool can_pull = g.symmetric || g.placement != GRUS_PLACE_CAPPED;  // needs the in-edges
   const uint64_t W = V / 64 + 1;
   for (;;) {
@@ -606,19 +593,37 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
       const uint32_t near_id = w.next_stamp(ctx, V);
       if (can_pull && mf > pull_edges) {
         const CsrView in = g.csc();
+        if (w.win_for != in.off) {
+          w.nwin = in.E / WIN + 1;
+          w.win_row.alloc((w.nwin + 1) * 4);
+          w.head_min.alloc(w.nwin * 4);
+          w.tail_min.alloc(w.nwin * 4);
+          build_window_rows(ctx, in.off, V, w.nwin, w.win_row.as<uint32_t>());
+          w.win_for = in.off;
+        }
         uint64_t *fbm = w.front_bm.as<uint64_t>();
         fill_u64(ctx, fbm, W, 0);
         queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nn, fbm, W);
-        sssp::k_pull_relax<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
-            in, fbm, dist, (uint32_t)std::min<uint64_t>(T_lo, 0xFFFFFFFFull), Tk,
-            w.near_stamp.as<uint32_t>(), near_id, w.far_stamp.as<uint32_t>(), far_id,
-            w.q[cur ^ 1].as<uint32_t>(), w.far[fcur].as<uint32_t>() + nfar,
-            w.longq.as<uint32_t>(), cnt, out.off);
+        sssp::PullMinOp op{};
+        op.in = in;
+        op.front = fbm;
+        op.dist = dist;
+        op.out_off = out.off;
+        op.T_thr = Tk;
+        op.near_stamp = w.near_stamp.as<uint32_t>();
+        op.near_id = near_id;
+        op.far_stamp = w.far_stamp.as<uint32_t>();
+        op.far_id = far_id;
+        op.head_min = w.head_min.as<uint32_t>();
+        op.tail_min = w.tail_min.as<uint32_t>();
+        op.qn = w.q[cur ^ 1].as<uint32_t>();
+        op.qfar = w.far[fcur].as<uint32_t>() + nfar;
+        op.cnt = cnt;
+        sssp::k_pull_min_window<<<(unsigned)w.nwin, BLOCK, 0, ctx.stream>>>(
+            in, w.win_row.as<uint32_t>(), op, w.nwin);
         GRUS_LAUNCH_CHECK();
-        sssp::k_pull_relax_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-            in, fbm, dist, Tk, w.near_stamp.as<uint32_t>(), near_id, w.far_stamp.as<uint32_t>(),
-            far_id, w.q[cur ^ 1].as<uint32_t>(), w.far[fcur].as<uint32_t>() + nfar,
-            w.longq.as<uint32_t>(), cnt, out.off);
+        sssp::k_pull_min_fixup<<<(unsigned)div_up(w.nwin, BLOCK), BLOCK, 0, ctx.stream>>>(
+            in, w.win_row.as<uint32_t>(), op, w.nwin);
         GRUS_LAUNCH_CHECK();
         launches += 4;
         pull_rounds++;
@@ -668,7 +673,6 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     mf = hc->next_edges;
     nfar = hc->far_count;
     fcur ^= 1;
-    T_lo = T;
     T = T_new;
   }
   GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));

@@ -104,8 +104,8 @@ typedef struct grus_tuning {
   uint32_t sssp_pull_div; /* SSSP: a round pulls (each unsettled vertex takes
                              the min over its in-neighbours in the frontier)
                              when the frontier's out-edges exceed
-                             E / sssp_pull_div; 0 = automatic (push
-                             only), 0xFFFFFFFF = push only */
+                             E / sssp_pull_div; 0 = automatic (16),
+                             0xFFFFFFFF = push only */
   uint32_t sssp_local_hops; /* SSSP on low-degree graphs (one persistent
                              launch): extra passes in which a workgroup
                              relaxes the vertices it has just improved before
