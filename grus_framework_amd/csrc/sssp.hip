@@ -523,7 +523,9 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   // few workgroups keep the barrier cheap; the frontier of a high-diameter
   // graph is small (GRUS_PERSIST_BLOCKS overrides for experiments)
   static const char *env_blocks = getenv("GRUS_PERSIST_BLOCKS");
-  unsigned want = env_blocks ? (unsigned)atoi(env_blocks) : (unsigned)std::max(1, ctx.cu_count / 8);
+  // CUs / 4 measured best on the 4096^2 grid (32..512 workgroups: 142, 134,
+  // 138, 147, 222 ms)
+  unsigned want = env_blocks ? (unsigned)atoi(env_blocks) : (unsigned)std::max(1, ctx.cu_count / 4);
   unsigned grid = std::max(1u, std::min(want, (unsigned)(ctx.cu_count * per_cu)));
   void *args[] = {&p};
   GRUS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sssp::k_persistent),
