@@ -328,8 +328,8 @@ __device__ __forceinline__ void publish(const uint32_t *lq, uint32_t lcount, uin
 }
 
 __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
-  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP];
-  __shared__ uint32_t s_nn, s_nf, s_base, s_hi;
+  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP], s_loc[PQ_CAP];
+  __shared__ uint32_t s_nn, s_nf, s_base, s_hi, s_nl;
   const CsrView g = p.g;
   unsigned int gen = 0;
   uint32_t step = 0, stamp = p.stamp0;
@@ -364,28 +364,37 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
           if (nd < Tk) {
             if (atomicExch(&p.near_stamp[v], near_id) != near_id)
               stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
+            // every improvement this workgroup makes is also worth a local
+            // hop, whoever publishes it (a full list only drops the hop)
+            if (p.local_hops) {
+              const uint32_t l = atomicAdd(&s_nl, 1u);
+              if (l < PQ_CAP) s_loc[l] = v;
+            }
           } else if (atomicExch(&p.far_stamp[v], far_id) != far_id) {
             stage(v, s_far, &s_nf, &C[1], far[fcur], nfar);
           }
         }
       };
+      if (threadIdx.x == 0) s_nl = 0;
+      __syncthreads();
       for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride)
         relax(ld_agent(&q[cur][i]));
       __syncthreads();
-      // local hops: the workgroup relaxes again the near vertices it has just
-      // generated (held in LDS), so the frontier advances several hops per
-      // grid barrier.  Everything staged is still published and relaxed
-      // again next round: extra relaxations never change the fixed point.
-      // The pass count is bounded and block-uniform: thread 0 snapshots the
-      // staged count between two barriers (others append to s_nn while a
-      // pass runs), so every thread sees the same bound and every workgroup
-      // reaches the grid barrier.
+      // local hops: the workgroup relaxes again the vertices it has just
+      // improved (held in LDS), so the frontier advances several hops per
+      // grid barrier.  Publishing is unchanged (stamps): every improved
+      // vertex is still relaxed in the next round, so the local passes only
+      // add relaxations and never change the fixed point.  The pass count
+      // is bounded and block-uniform: thread 0 snapshots the local count
+      // between two barriers (others append while a pass runs), so every
+      // thread sees the same bound and every workgroup reaches the grid
+      // barrier.
       for (uint32_t it = 0, lo = 0; it < p.local_hops; it++) {
-        if (threadIdx.x == 0) s_hi = s_nn < PQ_CAP ? s_nn : PQ_CAP;
+        if (threadIdx.x == 0) s_hi = s_nl < PQ_CAP ? s_nl : PQ_CAP;
         __syncthreads();
         const uint32_t hi = s_hi;
         if (hi <= lo) break;
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) relax(s_near[k]);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) relax(s_loc[k]);
         lo = hi;
         __syncthreads();  // the pass's appends done, and s_hi read by all
       }
@@ -510,11 +519,9 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   p.round_cap = std::min<uint64_t>(64 * V + 1024, 0x08000000ull);
   p.timeout = ctx.wall_ticks_per_ms() * 20000;  // 20 s per barrier wait
   p.stamp0 = w.stamp;
-  // one pass measured best on the 4096^2 grid (156 -> 143 ms; rounds 12.4K ->
-  // 6.5K): later passes find few new vertices, because the neighbours of a
-  // workgroup's frontier were mostly claimed by other workgroups, and each
-  // pass costs a chain of device-scope atomics
-  p.local_hops = ctx.tuning.sssp_local_hops ? ctx.tuning.sssp_local_hops : 1;
+  // 8 passes on the 4096^2 grid: rounds 6.5K -> 2.1K, 134 -> 130 ms; each
+  // pass costs a chain of device-scope atomics, so more passes gain little
+  p.local_hops = ctx.tuning.sssp_local_hops ? ctx.tuning.sssp_local_hops : 8;
   p.max_deg = (uint32_t)std::max<uint64_t>(g.max_deg, 1);
   int per_cu = 0;
   GRUS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(

@@ -109,7 +109,7 @@ typedef struct grus_tuning {
   uint32_t sssp_local_hops; /* SSSP on low-degree graphs (one persistent
                              launch): extra passes in which a workgroup
                              relaxes the vertices it has just improved before
-                             the grid barrier; 0 = automatic (1) */
+                             the grid barrier; 0 = automatic (8) */
   uint32_t reserved[4];
 } grus_tuning;
 
