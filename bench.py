@@ -400,7 +400,7 @@ def run_serving(grus):
            "serial_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
            "serial_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2)}
     # batched at saturation: all queries offered at once
-    with grus.Server(g, max_batch=64, max_inflight_batches=2) as srv:
+    with grus.Server(g, max_batch=64, max_inflight_batches=8) as srv:
         t0 = time.perf_counter()
         for i in range(N):
             srv.submit(int(kinds[i]), int(srcs[i]))
@@ -413,7 +413,7 @@ def run_serving(grus):
     rate = 0.6 * N / tb
     n2 = 1024
     gaps = rng.exponential(1.0 / rate, n2)
-    with grus.Server(g, max_batch=64, max_inflight_batches=2) as srv:
+    with grus.Server(g, max_batch=64, max_inflight_batches=8) as srv:
         t0 = time.perf_counter()
         due = t0
         for i in range(n2):

@@ -292,7 +292,7 @@ def autotune(g, kind="bfs", srcs=(), cache=None):
 class Server:
     """Continuous-batching query server (grus_server_*)."""
 
-    def __init__(self, g, max_inflight_batches=2, max_batch=64, state_budget_bytes=0,
+    def __init__(self, g, max_inflight_batches=4, max_batch=64, state_budget_bytes=0,
                  copy_results=False):
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,

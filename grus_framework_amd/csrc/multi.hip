@@ -165,9 +165,9 @@ __global__ __launch_bounds__(BLOCK) void k_bfs_finalize(const uint64_t *off, uin
                                                         uint64_t running, uint32_t t,
                                                         LaneStart st, uint32_t *queue,
                                                         MsCounters *cnt) {
-  __shared__ uint32_t s_cnt[64];
+  __shared__ unsigned long long s_alive;
   __shared__ uint32_t s_app[BLOCK / WAVE + 1];
-  for (int i = threadIdx.x; i < 64; i += blockDim.x) s_cnt[i] = 0;
+  if (threadIdx.x == 0) s_alive = 0;
   __syncthreads();
   uint64_t edges = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -185,13 +185,11 @@ __global__ __launch_bounds__(BLOCK) void k_bfs_finalize(const uint64_t *off, uin
       if (front[v] != nw) front[v] = nw;
     }
     uint64_t wm = wave_or(nw);
-    while (wm) {
+    if (lane_id() == 0 && wm) atomicOr(&s_alive, (unsigned long long)wm);
+    while (wm) {  // level rows: one coalesced 64-vertex store per lane present
       int i = __ffsll((unsigned long long)wm) - 1;
       wm &= wm - 1;
-      bool has = (nw >> i) & 1ull;
-      if (has) level[(uint64_t)i * V + v] = t + 1 - st.s[i];
-      uint64_t b = __ballot(has);
-      if (lane_id() == 0) atomicAdd(&s_cnt[i], (uint32_t)__popcll(b));
+      if ((nw >> i) & 1ull) level[(uint64_t)i * V + v] = t + 1 - st.s[i];
     }
     uint32_t slot = block_append(&cnt->qcount, nw != 0, s_app);
     if (nw) {
@@ -200,8 +198,7 @@ __global__ __launch_bounds__(BLOCK) void k_bfs_finalize(const uint64_t *off, uin
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 64; i += blockDim.x)
-    if (s_cnt[i]) atomicAdd(&cnt->lane_new[i], s_cnt[i]);
+  if (threadIdx.x == 0 && s_alive) atomicOr((unsigned long long *)&cnt->alive, s_alive);
   edges = wave_sum(edges);
   if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
 }
@@ -276,9 +273,9 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, ui
                                                          uint64_t *next, uint64_t V,
                                                          uint64_t running, uint32_t *queue,
                                                          MsCounters *cnt) {
-  __shared__ uint32_t s_cnt[64];
+  __shared__ unsigned long long s_alive;
   __shared__ uint32_t s_app[BLOCK / WAVE + 1];
-  for (int i = threadIdx.x; i < 64; i += blockDim.x) s_cnt[i] = 0;
+  if (threadIdx.x == 0) s_alive = 0;
   __syncthreads();
   uint64_t edges = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -291,13 +288,8 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, ui
       nw = nx & running;
       if (front[v] != nw) front[v] = nw;
     }
-    uint64_t wm = wave_or(nw);
-    while (wm) {
-      int i = __ffsll((unsigned long long)wm) - 1;
-      wm &= wm - 1;
-      uint64_t b = __ballot((nw >> i) & 1ull);
-      if (lane_id() == 0) atomicAdd(&s_cnt[i], (uint32_t)__popcll(b));
-    }
+    const uint64_t wm = wave_or(nw);
+    if (lane_id() == 0 && wm) atomicOr(&s_alive, (unsigned long long)wm);
     uint32_t slot = block_append(&cnt->qcount, nw != 0, s_app);
     if (nw) {
       queue[slot] = (uint32_t)v;
@@ -305,8 +297,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, ui
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 64; i += blockDim.x)
-    if (s_cnt[i]) atomicAdd(&cnt->lane_new[i], s_cnt[i]);
+  if (threadIdx.x == 0 && s_alive) atomicOr((unsigned long long *)&cnt->alive, s_alive);
   edges = wave_sum(edges);
   if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
 }
@@ -589,7 +580,7 @@ std::vector<int> MsEngine::step() {
   qn_ = h.qcount;
   front_edges_ = h.front_edges;
   for (int i = 0; i < lanes_; i++) {
-    if (((running >> i) & 1ull) && h.lane_new[i] == 0) {
+    if (((running >> i) & 1ull) && !((h.alive >> i) & 1ull)) {
       done.push_back(i);
       finished_mask_ |= 1ull << i;
     }

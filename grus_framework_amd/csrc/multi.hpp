@@ -25,7 +25,7 @@ void row_digest(Context &ctx, const uint32_t *row, uint64_t V, uint64_t *reached
                 uint64_t *checksum);
 
 struct MsCounters {
-  uint32_t lane_new[64];  // lanes' new frontier sizes this step
+  uint64_t alive;          // lanes whose new frontier is not empty this step
   uint64_t front_edges;   // out-edges of the new frontier (direction heuristic)
   uint32_t qcount;        // frontier queue length
   uint32_t long_count;    // pull: vertices handed to wave-cooperative scan

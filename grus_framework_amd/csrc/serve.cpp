@@ -40,7 +40,7 @@ struct Query {
 class ServerImpl : public Server {
  public:
   ServerImpl(Graph &g, const grus_server_opts *o) : g_(g) {
-    opts_.max_inflight_batches = 1;
+    opts_.max_inflight_batches = 4;
     opts_.max_batch = 64;
     opts_.state_budget_bytes = 0;
     opts_.copy_results = 0;

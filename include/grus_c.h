@@ -240,7 +240,10 @@ int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
 #define GRUS_Q_BFS 0
 #define GRUS_Q_SSSP 1
 typedef struct grus_server_opts {
-  int max_inflight_batches;   /* concurrent batches (streams), default 2 */
+  int max_inflight_batches;   /* concurrent batches per query kind (engines,
+                                 each with its own stream and worker thread),
+                                 1..8, default 4: more engines overlap more
+                                 of the per-step host round trips */
   int max_batch;              /* lanes per batch, 1..64, default 64 */
   uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
   int copy_results;           /* 1: copy each result to the host buffer given
