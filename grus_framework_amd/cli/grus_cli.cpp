@@ -94,6 +94,7 @@ void usage(const char *prog) {
           "  reference flags: --output FILE --src N --device N --ngpu 1 --pull\n"
           "                   --opt --pf --ab --rm --pl --wl_th X (accepted, unused)\n"
           "  placement:       --placement hbm|managed|capped --hbm_cap_gb G\n"
+          "  co-location:     --cu_limit N (run on the first N compute units)\n"
           "  inputs:          --symmetric --edge_factor N --seed N --weights LO,HI --wseed N\n"
           "  apps:            --direction auto|push|pull (bfs)  --iters N --alpha A (pr)\n"
           "                   --method hook|lp (cc)  --pr_unnormalized (dump ranks x V)\n",
@@ -136,6 +137,20 @@ int main(int argc, char **argv) {
 
   grus_ctx *ctx = nullptr;
   CHECK(grus_init((int)f.num("device", 0), &ctx));
+  if (f.has("cu_limit")) {  // the reference's SM-limit experiments, as a CU mask
+    int active = 0, total = 0;
+    CHECK(grus_cu_count(ctx, &active, &total));
+    const long n = f.num("cu_limit", total);
+    if (n < 1) {
+      fprintf(stderr, "--cu_limit: at least one compute unit\n");
+      return 2;
+    }
+    if (n < total) {
+      std::vector<uint32_t> mask((total + 31) / 32, 0u);
+      for (long i = 0; i < n; i++) mask[i / 32] |= 1u << (i % 32);
+      CHECK(grus_set_cu_mask(ctx, mask.data(), (uint32_t)mask.size()));
+    }
+  }
   grus_graph *g = nullptr;
   bool need_w = (app == "sssp");
   uint32_t wlo = 0, whi = 0;

@@ -118,6 +118,7 @@ int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in) {
     if (in->pr_layout > 3) mgg::raise(GRUS_E_INVALID, "pr_layout");
     if (in->pr_bin_bits != 0 && (in->pr_bin_bits < 6 || in->pr_bin_bits > 24))
       mgg::raise(GRUS_E_INVALID, "pr_bin_bits outside [6, 24]");
+    if (in->sssp_local_hops > 256) mgg::raise(GRUS_E_INVALID, "sssp_local_hops above 256");
     ctx->c->tuning = *in;
   });
 }

@@ -43,7 +43,8 @@ def test_cli_dump_matches_oracle(tmp_path, oracle, app):
 
 def test_cli_placement_flags(tmp_path, oracle):
     out = str(tmp_path / "o.txt")
-    for extra in (["--noopt", "--pf", "--ab"], ["--placement", "capped", "--hbm_cap_gb", "0.001"]):
+    for extra in (["--noopt", "--pf", "--ab"], ["--placement", "capped", "--hbm_cap_gb", "0.001"],
+                  ["--cu_limit", "16"]):
         r = subprocess.run([os.path.join(BIN, "grus_bfs"), "--rmat", "12", "--src", "0",
                             "--output", out] + extra, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
