@@ -529,10 +529,15 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   if (per_cu < 1) raise(GRUS_E_HIP, "persistent sssp kernel cannot be resident");
   // few workgroups keep the barrier cheap; the frontier of a high-diameter
   // graph is small (GRUS_PERSIST_BLOCKS overrides for experiments)
-  static const char *env_blocks = getenv("GRUS_PERSIST_BLOCKS");
+  // read once; the value is kept, not the environment's pointer (which a
+  // later change of the environment may free)
+  static const int env_blocks = [] {
+    const char *e = getenv("GRUS_PERSIST_BLOCKS");
+    return e ? atoi(e) : 0;
+  }();
   // CUs / 4 measured best on the 4096^2 grid (32..512 workgroups: 142, 134,
   // 138, 147, 222 ms)
-  unsigned want = env_blocks ? (unsigned)atoi(env_blocks) : (unsigned)std::max(1, ctx.cu_count / 4);
+  unsigned want = env_blocks > 0 ? (unsigned)env_blocks : (unsigned)std::max(1, ctx.cu_count / 4);
   unsigned grid = std::max(1u, std::min(want, (unsigned)(ctx.cu_count * per_cu)));
   void *args[] = {&p};
   GRUS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sssp::k_persistent),
