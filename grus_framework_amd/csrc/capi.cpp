@@ -115,10 +115,12 @@ int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in) {
     need(ctx, "ctx");
     need(in, "in");
     if (in->bfs_alpha < 0 || in->bfs_beta < 0) mgg::raise(GRUS_E_INVALID, "negative threshold");
-    if (in->pr_layout > 3) mgg::raise(GRUS_E_INVALID, "pr_layout");
+    if (in->pr_layout > 4) mgg::raise(GRUS_E_INVALID, "pr_layout");
     if (in->pr_bin_bits != 0 && (in->pr_bin_bits < 6 || in->pr_bin_bits > 24))
       mgg::raise(GRUS_E_INVALID, "pr_bin_bits outside [6, 24]");
     if (in->sssp_local_hops > 256) mgg::raise(GRUS_E_INVALID, "sssp_local_hops above 256");
+    if (in->pr_task_edges != 0 && in->pr_task_edges < 256)
+      mgg::raise(GRUS_E_INVALID, "pr_task_edges below 256");
     ctx->c->tuning = *in;
   });
 }

@@ -64,6 +64,19 @@ struct PrWork {
   uint64_t pb_G = 0;
   int pb_bits = 0;
   const void *pb_for = nullptr;  // offsets the layout was built on
+  // binned push layout (pr_layout = 4); see the comment above
+  // build_binned.  p1 = phase-1 stream; ch_u0 / ch_i0 = phase-1 chunks;
+  // dir = per (chunk, bin) slot offset; val = group values (bin-major);
+  // d2 = phase-2 edge stream (u16); gs16 = group index at every 16th edge;
+  // task / multi = phase-2 tasks and the bins split over several tasks;
+  // gacc = fixed-point sums of split bins (kept zero between iterations)
+  Buf bp_p1, bp_ch_u0, bp_ch_i0, bp_dir, bp_val, bp_d2, bp_gs16, bp_task, bp_multi, bp_gacc;
+  uint64_t bp_G = 0, bp_nch = 0, bp_ntask = 0, bp_nmulti = 0;
+  uint32_t bp_nbins = 0;
+  int bp_bits = 0;
+  const void *bp_for = nullptr;
+  std::vector<uint32_t> bp_bstart;  // first phase-2 edge of every bin (host)
+  uint64_t bp_task_edges = 0;       // task size the task table was cut with
 };
 
 __global__ void k_window_rows(const uint64_t *off, uint64_t V, uint64_t nwin, uint32_t *win_row) {
@@ -237,7 +250,7 @@ __device__ __forceinline__ float finish_row(float sum, float base, float alpha) 
 
 // a completed row writes its rank (last iteration) or next contrib
 template <bool kLast>
-__device__ __forceinline__ void finish(uint32_t r, float s, const uint64_t *out_off, float *next,
+__device__ __forceinline__ void finish(uint64_t r, float s, const uint64_t *out_off, float *next,
                                        float base, float alpha) {
   const float rank = finish_row(s, base, alpha);
   if (kLast) {
@@ -246,6 +259,228 @@ __device__ __forceinline__ void finish(uint32_t r, float s, const uint64_t *out_
     const uint64_t d = out_off[r + 1] - out_off[r];
     next[r] = d ? rank / (float)d : 0.0f;
   }
+}
+
+// ---- binned push layout (pr_layout = 4), per iteration ----
+constexpr int BP_MAX_BITS = 14;       // phase-2 accumulators: 2^14 x u64 = 128 KB of LDS
+constexpr uint32_t BP_MAX_BINS = 8192;  // phase-1 directory row in LDS: <= 32 KB
+constexpr int BP_BLOCK = 1024;        // phase-2 workgroup (one per CU: its LDS)
+constexpr int BP_EPT = 16;            // phase-2 edges per thread and step (32-byte load)
+constexpr int BP_FIX = 62;            // sums are kept as floor(x * 2^62) in u64
+
+struct BpTask {
+  uint32_t e0, e1;  // edge range of the phase-2 stream
+  uint32_t bin;
+  uint32_t single;  // 1: the bin's only task, which finishes its rows
+};
+
+// floor(x * 2^BP_FIX) for a finite x in [0, 4): contributions are ranks
+// divided by degrees, and ranks sum to at most 1
+__device__ __forceinline__ unsigned long long to_fixed(float x) {
+  const uint32_t b = __float_as_uint(x);
+  const int e = (int)((b >> 23) & 0xFFu);
+  const unsigned long long m = (b & 0x7FFFFFu) | (e ? 0x800000u : 0u);
+  int sh = (e ? e : 1) - 150 + BP_FIX;  // x = m * 2^((e ? e : 1) - 150)
+  if (sh > 40) sh = 40;                 // x >= 4 is outside the domain (no UB shift)
+  if (sh >= 0) return m << sh;
+  return sh > -64 ? m >> -sh : 0ull;
+}
+
+__device__ __forceinline__ float from_fixed(unsigned long long s) {
+  return (float)((double)s * 0x1p-62);
+}
+
+// Phase 1: val[slot(j)] = contrib[src(j)] for every group j.  Workgroup c
+// takes the groups of one source chunk in (bin, source) order, so the
+// stores of each bin are one contiguous run; dir[c][bin] turns a stream
+// position into its slot (u32 arithmetic, slots < 2^32).
+__global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, const uint32_t *ch_i0,
+                                                      const uint32_t *p1, const uint32_t *dir,
+                                                      uint32_t nbins, const float *contrib,
+                                                      float *val) {
+  extern __shared__ uint32_t sdir[];
+  const uint64_t c = blockIdx.x;
+  const uint32_t *row = dir + c * nbins;
+  for (uint32_t b = threadIdx.x; b < nbins; b += BLOCK) sdir[b] = row[b];
+  __syncthreads();
+  const uint32_t i0 = ch_i0[c], i1 = ch_i0[c + 1];
+  const float *cu = contrib + ch_u0[c];
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    const uint32_t w = __builtin_nontemporal_load(&p1[i]);
+    val[sdir[w >> 16] + i] = cu[w & 0xFFFFu];
+  }
+}
+
+// Phase 2: one task = a piece of one destination bin's edge stream.  Each
+// stream entry is a u16: bit 15 starts a group, bits 0..14 the destination
+// within the bin.  Threads take 16 entries (one 32-byte load) and the group
+// index of the first from gs16; the values are added into LDS accumulators
+// as fixed-point integers, so the sums do not depend on the order of the
+// atomics.  A bin's only task finishes its rows; pieces of split bins add
+// their sums into gacc, which k_bp_finish completes.
+template <bool kLast>
+__global__ __launch_bounds__(BP_BLOCK) void k_bp_accum(
+    const BpTask *tasks, const uint16_t *d2, const uint32_t *gs16, const float *val, uint64_t V,
+    int bits, const uint64_t *out_off, float *next, unsigned long long *gacc, float base,
+    float alpha) {
+  __shared__ unsigned long long acc[1 << BP_MAX_BITS];
+  const BpTask t = tasks[blockIdx.x];
+  const uint64_t v0 = (uint64_t)t.bin << bits;
+  const uint32_t nb = (uint32_t)((V - v0) < (1ull << bits) ? (V - v0) : (1ull << bits));
+  for (uint32_t i = threadIdx.x; i < nb; i += BP_BLOCK) acc[i] = 0ull;
+  __syncthreads();
+  const uint32_t k1 = (uint32_t)((t.e1 + (uint64_t)BP_EPT - 1) / BP_EPT);
+  for (uint32_t k = t.e0 / BP_EPT + threadIdx.x; k < k1; k += BP_BLOCK) {
+    const u32x4 *p = reinterpret_cast<const u32x4 *>(d2 + (uint64_t)k * BP_EPT);
+    const u32x4 a = __builtin_nontemporal_load(p);
+    const u32x4 b = __builtin_nontemporal_load(p + 1);
+    const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t g = gs16[k] - 1u;  // group before the chunk's first entry
+    uint32_t gq[BP_EPT], dq[BP_EPT];
+    const uint32_t p0 = k * BP_EPT;
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) {
+      const uint32_t e = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
+      g += e >> 15;
+      gq[q] = g;
+      dq[q] = e & 0x7FFFu;
+    }
+    float x[BP_EPT];
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) {
+      const uint32_t pe = p0 + q;
+      x[q] = (pe >= t.e0 && pe < t.e1) ? val[gq[q]] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) {
+      const uint32_t pe = p0 + q;
+      if (pe >= t.e0 && pe < t.e1) atomicAdd(&acc[dq[q]], to_fixed(x[q]));
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nb; i += BP_BLOCK) {
+    const unsigned long long s = acc[i];
+    if (t.single) finish<kLast>(v0 + i, from_fixed(s), out_off, next, base, alpha);
+    else if (s) atomicAdd(&gacc[v0 + i], s);
+  }
+}
+
+// rows of the bins split over several tasks; clears gacc for the next use
+template <bool kLast>
+__global__ __launch_bounds__(BLOCK) void k_bp_finish(const uint32_t *mbins, int bits, uint64_t V,
+                                                     unsigned long long *gacc,
+                                                     const uint64_t *out_off, float *next,
+                                                     float base, float alpha) {
+  const uint32_t per = (1u << bits) / BLOCK > 0 ? (1u << bits) / BLOCK : 1u;
+  const uint64_t bin = mbins[blockIdx.x / per];
+  const uint64_t i = (uint64_t)(blockIdx.x % per) * BLOCK + threadIdx.x;
+  const uint64_t v = (bin << bits) + i;
+  if (i >= (1ull << bits) || v >= V) return;
+  const unsigned long long s = gacc[v];
+  gacc[v] = 0ull;
+  finish<kLast>(v, from_fixed(s), out_off, next, base, alpha);
+}
+
+// ---- binned push layout construction ----
+
+// keys of the phase-1 order: (chunk << 16 | bin) of every group, with
+// chunk = the chunk whose group range holds j
+__global__ void k_bp_p1_keys(const uint64_t *gkey, uint64_t G, const uint32_t *ch_i0,
+                             uint32_t nch, uint64_t *key, uint32_t *ids) {
+  uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; j < G; j += stride) {
+    uint32_t lo = 0, hi = nch;  // ch_i0[lo] <= j < ch_i0[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ch_i0[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    key[j] = ((uint64_t)lo << 16) | (gkey[j] >> 32);
+    ids[j] = (uint32_t)j;
+  }
+}
+
+// group keys by bin only (stable sort keeps the source order inside a bin)
+__global__ void k_bp_bin_keys(const uint64_t *gkey, uint64_t G, uint64_t *key, uint32_t *ids) {
+  uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; j < G; j += stride) {
+    key[j] = gkey[j] >> 32;
+    ids[j] = (uint32_t)j;
+  }
+}
+
+// slot[group] = its position in (bin, source) order
+__global__ void k_bp_slots(const uint32_t *sorted, uint64_t G, uint32_t *slot) {
+  uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; s < G; s += stride) slot[sorted[s]] = (uint32_t)s;
+}
+
+// phase-1 stream entry and directory: position i of chunk c holds group
+// sorted[i]; the first position of each (chunk, bin) run records
+// slot - position
+__global__ void k_bp_p1(const uint32_t *sorted, const uint64_t *gkey, const uint32_t *slot,
+                        uint64_t G, const uint32_t *ch_u0, const uint32_t *ch_i0, uint32_t nch,
+                        uint32_t nbins, uint32_t *p1, uint32_t *dir) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < G; i += stride) {
+    uint32_t lo = 0, hi = nch;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ch_i0[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t j = sorted[i];
+    const uint32_t bin = (uint32_t)(gkey[j] >> 32);
+    const uint32_t src = (uint32_t)gkey[j];
+    p1[i] = (bin << 16) | (src - ch_u0[lo]);
+    const bool run_start = (i == ch_i0[lo]) || (uint32_t)(gkey[sorted[i - 1]] >> 32) != bin;
+    if (run_start) dir[(uint64_t)lo * nbins + bin] = slot[j] - (uint32_t)i;
+  }
+}
+
+// phase-2 stream keys: CSR edge e -> (bin of its destination, entry)
+__global__ void k_bp_d2_keys(const uint64_t *off, const uint32_t *col, uint64_t V, int bits,
+                             const uint32_t *flag, uint64_t *key, uint32_t *ent) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint32_t mask = (1u << bits) - 1u;
+  for (uint64_t u = wave; u < V; u += nw)
+    for (uint64_t e = off[u] + lane_id(); e < off[u + 1]; e += WAVE) {
+      const uint32_t v = col[e];
+      key[e] = v >> bits;
+      ent[e] = (flag[e] << 15) | (v & mask);
+    }
+}
+
+// d2 (u16, padded with zeros to a multiple of BP_EPT), its group flags as
+// u32 for the scan, and the first edge of every bin
+__global__ void k_bp_d2(const uint64_t *key, const uint32_t *ent, uint64_t E, uint64_t Epad,
+                        uint32_t nbins, uint16_t *d2, uint32_t *f, uint32_t *bstart) {
+  uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; p < Epad; p += stride) {
+    if (p >= E) {
+      d2[p] = 0;
+      continue;
+    }
+    d2[p] = (uint16_t)ent[p];
+    f[p] = ent[p] >> 15;
+    const uint64_t kb = key[p];
+    const uint64_t ka = p ? key[p - 1] + 1 : 0;  // bins (key[p-1], key[p]] start here
+    for (uint64_t b = ka; b <= kb; b++) bstart[b] = (uint32_t)p;
+    if (p == E - 1)
+      for (uint64_t b = kb + 1; b <= nbins; b++) bstart[b] = (uint32_t)E;
+  }
+}
+
+__global__ void k_bp_gs16(const uint32_t *excl, uint64_t E, uint64_t nk, uint32_t *gs16) {
+  uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; k < nk; k += stride) gs16[k] = excl[k * BP_EPT < E ? k * BP_EPT : E - 1];
 }
 
 // PageRank as a window reduction: sum of contrib[u] over the in-edges
@@ -415,6 +650,284 @@ bool build_blocked(Graph &g, PrWork &w, int bits) {
   w.pb_for = out.off;
   return true;
 }
+
+// Binned push layout (pr_layout = 4), built once per graph and bin width.
+// Destinations are cut into bins of 2^bits vertices (bits <= 14) and the
+// out-edges of each source into groups: maximal runs whose destinations
+// share a bin.  All edges of a group carry the same value contrib[u].
+//   phase 1  writes one value per group into val[], bin-major and by
+//            source inside a bin.  Sources are cut into chunks of at most
+//            32K groups and 64K sources; a chunk's groups are streamed in
+//            (bin, source) order (p1: bin << 16 | source - first source),
+//            so each bin receives one contiguous run per chunk.
+//   phase 2  streams each bin's edges in (source, destination) order as
+//            u16 entries (group-start flag, destination in the bin) next to
+//            the bin's slice of val[], and adds the values into LDS
+//            accumulators.  Bins with more than BP_TASK_EDGES edges are
+//            split into several tasks whose partial sums meet in gacc.
+// Every access of both phases is a sequential stream except the LDS
+// atomics, the reads of contrib inside one chunk's source range and the
+// per-bin runs of phase 1.  Returns false (the plain pull takes over) when
+// the graph does not fit the layout or the device lacks memory for the
+// build (peak ~ 40E + 16V bytes).
+constexpr uint32_t BP_CHUNK_GROUPS = 32768;
+constexpr uint64_t BP_SPAN = 65536;
+constexpr uint64_t BP_TASK_EDGES = 1ull << 18;  // automatic task size
+
+bool build_binned(Graph &g, PrWork &w, int bits) {
+  Context &ctx = *g.ctx;
+  const CsrView out = g.csr();
+  const uint64_t V = g.V, E = g.E;
+  const uint64_t nbins = div_up(V, 1ull << bits);
+  if (E == 0 || E >= (1ull << 32) - 64 || V >= (1ull << 32) || nbins > pr::BP_MAX_BINS)
+    return false;
+  size_t free_b = 0, total_b = 0;
+  GRUS_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (free_b < 48 * E + 16 * V + (1ull << 30)) return false;
+  for (Buf *b : {&w.bp_p1, &w.bp_ch_u0, &w.bp_ch_i0, &w.bp_dir, &w.bp_val, &w.bp_d2, &w.bp_gs16,
+                 &w.bp_task, &w.bp_multi, &w.bp_gacc})
+    b->release();
+  w.bp_for = nullptr;
+  const unsigned rows_grid = grid_cap(ctx, V * WAVE, BLOCK);
+  Buf flag, excl;
+  flag.alloc(E * 4);
+  excl.alloc(E * 4);
+  pr::k_pb_flags<<<rows_grid, BLOCK, 0, ctx.stream>>>(out.off, out.col, V, bits,
+                                                      flag.as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  scan_excl_u32(ctx, flag.as<uint32_t>(), excl.as<uint32_t>(), E);
+  uint32_t last[2];
+  GRUS_HIP(hipMemcpyAsync(&last[0], excl.as<uint32_t>() + E - 1, 4, hipMemcpyDeviceToHost,
+                          ctx.stream));
+  GRUS_HIP(hipMemcpyAsync(&last[1], flag.as<uint32_t>() + E - 1, 4, hipMemcpyDeviceToHost,
+                          ctx.stream));
+  ctx.sync();
+  const uint64_t G = (uint64_t)last[0] + last[1];
+  // groups per source (goff) -> phase-1 chunks, on the host
+  std::vector<uint32_t> ch_u0, ch_i0;
+  {
+    Buf goff;
+    goff.alloc((V + 1) * 8);
+    pr::k_pb_goff<<<grid_cap(ctx, V + 1, BLOCK), BLOCK, 0, ctx.stream>>>(
+        out.off, V, E, excl.as<uint32_t>(), G, goff.as<uint64_t>());
+    GRUS_LAUNCH_CHECK();
+    std::vector<uint64_t> h(V + 1);
+    GRUS_HIP(hipMemcpyAsync(h.data(), goff.p, (V + 1) * 8, hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    uint64_t u0 = 0;
+    while (u0 < V) {
+      uint64_t u1 = u0 + 1;  // a source has at most nbins <= 8192 groups
+      while (u1 < V && u1 - u0 < BP_SPAN && h[u1 + 1] - h[u0] <= BP_CHUNK_GROUPS) u1++;
+      ch_u0.push_back((uint32_t)u0);
+      ch_i0.push_back((uint32_t)h[u0]);
+      u0 = u1;
+    }
+    ch_i0.push_back((uint32_t)G);
+  }
+  const uint32_t nch = (uint32_t)ch_u0.size();
+  w.bp_ch_u0.alloc(nch * 4);
+  w.bp_ch_i0.alloc((nch + 1) * 4);
+  GRUS_HIP(hipMemcpyAsync(w.bp_ch_u0.p, ch_u0.data(), nch * 4, hipMemcpyHostToDevice, ctx.stream));
+  GRUS_HIP(hipMemcpyAsync(w.bp_ch_i0.p, ch_i0.data(), (nch + 1) * 4, hipMemcpyHostToDevice,
+                          ctx.stream));
+  // group keys (bin << 32 | source), by group id
+  Buf gkey, slot;
+  gkey.alloc(G * 8);
+  pr::k_pb_gkeys<<<rows_grid, BLOCK, 0, ctx.stream>>>(out.off, out.col, V, bits,
+                                                      flag.as<uint32_t>(), excl.as<uint32_t>(),
+                                                      gkey.as<uint64_t>());
+  GRUS_LAUNCH_CHECK();
+  slot.alloc(G * 4);
+  {  // slots: (bin, source) order
+    Buf ka, kb, va, vb;
+    ka.alloc(G * 8);
+    kb.alloc(G * 8);
+    va.alloc(G * 4);
+    vb.alloc(G * 4);
+    pr::k_bp_bin_keys<<<grid_cap(ctx, G, BLOCK), BLOCK, 0, ctx.stream>>>(
+        gkey.as<uint64_t>(), G, ka.as<uint64_t>(), va.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    uint32_t *sorted = nullptr;
+    sort_pairs_u64_u32(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<uint32_t>(),
+                       vb.as<uint32_t>(), G, &sorted, std::max(1, bits_for(nbins - 1)));
+    pr::k_bp_slots<<<grid_cap(ctx, G, BLOCK), BLOCK, 0, ctx.stream>>>(sorted, G,
+                                                                     slot.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    ctx.sync();
+  }
+  w.bp_p1.alloc(G * 4);
+  w.bp_dir.alloc((uint64_t)nch * nbins * 4);
+  GRUS_HIP(hipMemsetAsync(w.bp_dir.p, 0, (uint64_t)nch * nbins * 4, ctx.stream));
+  {  // phase-1 stream: (chunk, bin, source) order
+    Buf ka, kb, va, vb;
+    ka.alloc(G * 8);
+    kb.alloc(G * 8);
+    va.alloc(G * 4);
+    vb.alloc(G * 4);
+    pr::k_bp_p1_keys<<<grid_cap(ctx, G, BLOCK), BLOCK, 0, ctx.stream>>>(
+        gkey.as<uint64_t>(), G, w.bp_ch_i0.as<uint32_t>(), nch, ka.as<uint64_t>(),
+        va.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    uint32_t *sorted = nullptr;
+    sort_pairs_u64_u32(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<uint32_t>(),
+                       vb.as<uint32_t>(), G, &sorted, 16 + bits_for(nch));
+    pr::k_bp_p1<<<grid_cap(ctx, G, BLOCK), BLOCK, 0, ctx.stream>>>(
+        sorted, gkey.as<uint64_t>(), slot.as<uint32_t>(), G, w.bp_ch_u0.as<uint32_t>(),
+        w.bp_ch_i0.as<uint32_t>(), nch, (uint32_t)nbins, w.bp_p1.as<uint32_t>(),
+        w.bp_dir.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    ctx.sync();
+  }
+  gkey.release();
+  slot.release();
+  // phase-2 stream: CSR edges stably sorted by destination bin
+  const uint64_t Epad = div_up(E, pr::BP_EPT) * pr::BP_EPT;
+  const uint64_t nk = Epad / pr::BP_EPT;
+  std::vector<uint32_t> bstart(nbins + 1);
+  w.bp_d2.alloc(Epad * 2);
+  w.bp_gs16.alloc(nk * 4);
+  {
+    Buf ka, kb, vb, bs;
+    ka.alloc(E * 8);
+    kb.alloc(E * 8);
+    vb.alloc(E * 4);
+    bs.alloc((nbins + 1) * 4);
+    pr::k_bp_d2_keys<<<rows_grid, BLOCK, 0, ctx.stream>>>(out.off, out.col, V, bits,
+                                                         flag.as<uint32_t>(), ka.as<uint64_t>(),
+                                                         excl.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    uint32_t *ent = nullptr;
+    uint64_t *key = sort_pairs_u64_u32(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(),
+                                       excl.as<uint32_t>(), vb.as<uint32_t>(), E, &ent,
+                                       std::max(1, bits_for(nbins - 1)));
+    // flags of the sorted stream go to whichever entry buffer is free
+    uint32_t *f = (ent == excl.as<uint32_t>()) ? vb.as<uint32_t>() : excl.as<uint32_t>();
+    pr::k_bp_d2<<<grid_cap(ctx, Epad, BLOCK), BLOCK, 0, ctx.stream>>>(
+        key, ent, E, Epad, (uint32_t)nbins, w.bp_d2.as<uint16_t>(), f, bs.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    scan_excl_u32(ctx, f, flag.as<uint32_t>(), E);  // flag now holds the exclusive scan
+    pr::k_bp_gs16<<<grid_cap(ctx, nk, BLOCK), BLOCK, 0, ctx.stream>>>(
+        flag.as<uint32_t>(), E, nk, w.bp_gs16.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    GRUS_HIP(hipMemcpyAsync(bstart.data(), bs.p, (nbins + 1) * 4, hipMemcpyDeviceToHost,
+                            ctx.stream));
+    ctx.sync();
+  }
+  flag.release();
+  excl.release();
+  w.bp_val.alloc(G * 4);
+  ctx.sync();  // host vectors above are copied before they go out of scope
+  w.bp_bstart = std::move(bstart);
+  w.bp_task_edges = 0;
+  w.bp_G = G;
+  w.bp_nch = nch;
+  w.bp_nbins = (uint32_t)nbins;
+  w.bp_bits = bits;
+  w.bp_for = out.off;
+  return true;
+}
+
+// phase-2 tasks: every bin's edge range cut into equal pieces of at most
+// task_edges edges (an empty bin still gets one task: its rows are finished)
+void build_binned_tasks(Graph &g, PrWork &w, uint64_t task_edges) {
+  Context &ctx = *g.ctx;
+  std::vector<pr::BpTask> tasks;
+  std::vector<uint32_t> multi;
+  for (uint64_t b = 0; b < w.bp_nbins; b++) {
+    const uint64_t e0 = w.bp_bstart[b], e1 = w.bp_bstart[b + 1];
+    const uint64_t n = e1 - e0;
+    const uint64_t parts = n > task_edges ? div_up(n, task_edges) : 1;
+    if (parts > 1) multi.push_back((uint32_t)b);
+    for (uint64_t q = 0; q < parts; q++)
+      tasks.push_back(pr::BpTask{(uint32_t)(e0 + n * q / parts),
+                                 (uint32_t)(e0 + n * (q + 1) / parts), (uint32_t)b,
+                                 parts == 1 ? 1u : 0u});
+  }
+  w.bp_task.alloc(tasks.size() * sizeof(pr::BpTask));
+  GRUS_HIP(hipMemcpyAsync(w.bp_task.p, tasks.data(), tasks.size() * sizeof(pr::BpTask),
+                          hipMemcpyHostToDevice, ctx.stream));
+  w.bp_multi.release();
+  if (!multi.empty()) {
+    w.bp_multi.alloc(multi.size() * 4);
+    GRUS_HIP(hipMemcpyAsync(w.bp_multi.p, multi.data(), multi.size() * 4, hipMemcpyHostToDevice,
+                            ctx.stream));
+    if (!w.bp_gacc.p) {
+      w.bp_gacc.alloc(g.V * 8);
+      GRUS_HIP(hipMemsetAsync(w.bp_gacc.p, 0, g.V * 8, ctx.stream));
+    }
+  }
+  ctx.sync();  // the host vectors are copied before they go out of scope
+  w.bp_ntask = tasks.size();
+  w.bp_nmulti = multi.size();
+  w.bp_task_edges = task_edges;
+}
+
+// iterations over the binned push layout (graph already built for it)
+void run_binned(Graph &g, PrWork &w, float alpha, int iters, grus_stats *st) {
+  Context &ctx = *g.ctx;
+  const uint64_t V = g.V;
+  const CsrView out = g.csr();
+  w.contrib[0].ensure(V * 4);
+  g.result.ensure(V * 4);
+  float *rank = g.result.as<float>();
+  const float base = (float)((1.0 - (double)alpha) / (double)V);
+  const int bits = w.bp_bits;
+  const unsigned fin_grid =
+      (unsigned)(w.bp_nmulti * std::max<uint64_t>(1, (1ull << bits) / BLOCK));
+  float *contrib = w.contrib[0].as<float>();
+  const pr::BpTask *tasks = w.bp_task.as<const pr::BpTask>();
+  unsigned long long *gacc = w.bp_gacc.as<unsigned long long>();
+  GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  pr::k_init_contrib<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(out.off, V, base, contrib);
+  GRUS_LAUNCH_CHECK();
+  uint64_t launches = 1;
+  if (iters == 0) {  // r0 = (1-a)/V everywhere
+    uint32_t b;
+    memcpy(&b, &base, 4);
+    fill_u32(ctx, reinterpret_cast<uint32_t *>(rank), V, b);
+    launches++;
+  }
+  for (int it = 0; it < iters; it++) {
+    const bool last = (it == iters - 1);
+    // phase 1 reads contrib; phase 2 overwrites it (or writes the ranks)
+    float *next = last ? rank : contrib;
+    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, w.bp_nbins * 4, ctx.stream>>>(
+        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_p1.as<uint32_t>(),
+        w.bp_dir.as<uint32_t>(), w.bp_nbins, contrib, w.bp_val.as<float>());
+    GRUS_LAUNCH_CHECK();
+    if (last) {
+      pr::k_bp_accum<true><<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
+          tasks, w.bp_d2.as<uint16_t>(), w.bp_gs16.as<uint32_t>(), w.bp_val.as<float>(), V, bits,
+          out.off, next, gacc, base, alpha);
+      GRUS_LAUNCH_CHECK();
+      if (w.bp_nmulti)
+        pr::k_bp_finish<true><<<fin_grid, BLOCK, 0, ctx.stream>>>(
+            w.bp_multi.as<uint32_t>(), bits, V, gacc, out.off, next, base, alpha);
+    } else {
+      pr::k_bp_accum<false><<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
+          tasks, w.bp_d2.as<uint16_t>(), w.bp_gs16.as<uint32_t>(), w.bp_val.as<float>(), V, bits,
+          out.off, next, gacc, base, alpha);
+      GRUS_LAUNCH_CHECK();
+      if (w.bp_nmulti)
+        pr::k_bp_finish<false><<<fin_grid, BLOCK, 0, ctx.stream>>>(
+            w.bp_multi.as<uint32_t>(), bits, V, gacc, out.off, next, base, alpha);
+    }
+    GRUS_LAUNCH_CHECK();
+    launches += w.bp_nmulti ? 3 : 2;
+  }
+  GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  GRUS_HIP(hipEventSynchronize(ctx.ev1));
+  float ms = 0;
+  GRUS_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+  if (st) {
+    *st = grus_stats{};
+    st->device_ms = ms;
+    st->iterations = (uint32_t)iters;
+    st->vertices_reached = V;
+    st->edges_reached = g.E;
+    st->kernel_launches = launches;
+  }
+}
 }  // namespace
 
 void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
@@ -428,13 +941,26 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
   const uint64_t V = g.V;
   const uint32_t layout = ctx.tuning.pr_layout;
   const int bits = ctx.tuning.pr_bin_bits ? (int)ctx.tuning.pr_bin_bits : PB_BITS;
-  if (layout > 3 || bits < 6 || bits > 24) raise(GRUS_E_INVALID, "PageRank layout tuning");
+  if (layout > 4 || bits < 6 || bits > 24) raise(GRUS_E_INVALID, "PageRank layout tuning");
+  CsrView out = g.csr();
+  // binned push (automatic choice on HBM graphs that fit it; bins of at
+  // most 2^14 destinations, the LDS accumulators)
+  if ((layout == 0 || layout == 4) && g.placement == GRUS_PLACE_HBM) {
+    const int bb = std::min(bits, pr::BP_MAX_BITS);
+    bool binned = true;
+    if (w.bp_for != out.off || w.bp_bits != bb) binned = build_binned(g, w, bb);
+    const uint64_t te = ctx.tuning.pr_task_edges ? ctx.tuning.pr_task_edges : BP_TASK_EDGES;
+    if (binned && w.bp_task_edges != te) build_binned_tasks(g, w, te);
+    if (binned) {
+      run_binned(g, w, alpha, iters, st);
+      return;
+    }
+  }
   // relabelling only on request: on RMAT inputs the natural numbering
   // already spreads the hubs over many cache lines, while degree order packs
   // the hottest contributions into a few lines and measured ~5x slower
   const bool relabel = g.symmetric && g.placement == GRUS_PLACE_HBM && layout == 1;
   if (relabel && !w.relabelled) build_relabelled(g, w);
-  CsrView out = g.csr();
   CsrView in = g.csc();
   // blocked layout on request only (HBM graphs whose group and edge ids fit
   // 32 bits): on RMAT s24 its scatter costs more than the gathers it saves

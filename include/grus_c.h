@@ -92,15 +92,21 @@ typedef struct grus_tuning {
   uint32_t bu_thread_edges; /* bottom-up: edges a lane scans before handing
                                the vertex to a wave (default 32) */
   uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic) */
-  uint32_t pr_layout;     /* PageRank edge layout: 0 automatic (= 2); 1 degree-
-                             sorted relabelled copy (symmetric HBM graphs);
-                             2 plain pull over the in-edges; 3 blocked
-                             (propagation blocking over bins of 2^pr_bin_bits
-                             destinations; HBM graphs with E <= 2^31, else 2).
-                             Layouts 2 and 3 give bit-identical ranks; the
-                             relabelled copy sums in another order (same
-                             ranks up to float rounding). */
-  uint32_t pr_bin_bits;   /* blocked layout bin width, log2 (0 = automatic) */
+  uint32_t pr_layout;     /* PageRank edge layout: 0 automatic (= 4 where it
+                             applies, else 2); 1 degree-sorted relabelled copy
+                             (symmetric HBM graphs); 2 plain pull over the
+                             in-edges; 3 blocked (propagation blocking over
+                             bins of 2^pr_bin_bits destinations; HBM graphs
+                             with E <= 2^31, else 2); 4 binned push (values
+                             streamed per destination bin of at most 2^14
+                             vertices and summed in LDS as 64-bit fixed point;
+                             HBM graphs with E < 2^32 and at most 8192 bins,
+                             else 2).  Layouts 2 and 3 give bit-identical
+                             ranks; 1 and 4 sum in another order or precision
+                             (same ranks up to float rounding).  Every layout
+                             is bit-identical run to run. */
+  uint32_t pr_bin_bits;   /* blocked / binned layout bin width, log2 (0 =
+                             automatic: 14; layout 4 caps it at 14) */
   uint32_t sssp_pull_div; /* SSSP: a round pulls (each unsettled vertex takes
                              the min over its in-neighbours in the frontier)
                              when the frontier's out-edges exceed
@@ -110,7 +116,10 @@ typedef struct grus_tuning {
                              launch): extra passes in which a workgroup
                              relaxes the vertices it has just improved before
                              the grid barrier; 0 = automatic (8) */
-  uint32_t reserved[4];
+  uint32_t pr_task_edges; /* binned PageRank layout: edges per accumulation
+                             task; a bin with more edges is split over several
+                             tasks (0 = automatic, 2^18) */
+  uint32_t reserved[3];
 } grus_tuning;
 
 /* ---- autotuning (no reference counterpart: the reference hard-codes its

@@ -222,12 +222,13 @@ def test_pagerank_relabelled_path(ctx, oracle):
         assert np.abs(r - oracle.pagerank(h)).sum() <= PR_TOL
 
 
-def pagerank_with(ctx, g, layout, bits=0, iters=20):
+def pagerank_with(ctx, g, layout, bits=0, iters=20, task_edges=0):
     t0 = ctx.tuning
     try:
         t = ctx.tuning
         t.pr_layout = layout
         t.pr_bin_bits = bits
+        t.pr_task_edges = task_edges
         ctx.tuning = t
         return grus.pagerank(g, 0.85, iters)
     finally:
@@ -293,6 +294,69 @@ def test_pagerank_ragged_in_rows(ctx, oracle):
         print("ragged V=%d E=%d L1 %.3e" % (V, g.E, np.abs(a - ref).sum()), flush=True)
         assert np.abs(a.astype(np.float64) - ref).sum() <= PR_TOL
         assert (a.view(np.uint32) == b.view(np.uint32)).all()
+
+
+# ------------------------------------------------ PageRank, binned push layout
+@pytest.mark.parametrize("bits,task_edges", [(14, 0), (10, 256), (6, 5000), (14, 300)])
+def test_pagerank_binned_vs_fp64(ctx, oracle, bits, task_edges):
+    # symmetric RMAT and a directed graph; tiny task sizes split every large
+    # bin over several tasks (sums meet in the global accumulator)
+    rng = np.random.default_rng(12)
+    off, col, _ = random_directed(rng, 30000, 400000)
+    with grus.Graph.rmat(ctx, 15) as g, \
+            grus.Graph.from_csr(ctx, off, col, symmetric=False) as d:
+        for gr in (g, d):
+            h = host_csr(gr)
+            csc = None if gr is g else oracle.transpose(h)
+            ref = oracle.pagerank(h, 0.85, 20, csc=csc)
+            a, _ = pagerank_with(ctx, gr, 2)
+            b, st = pagerank_with(ctx, gr, 4, bits, task_edges=task_edges)
+            c, _ = pagerank_with(ctx, gr, 4, bits, task_edges=task_edges)
+            l1 = np.abs(b.astype(np.float64) - ref).sum()
+            print("binned bits %d task %d V=%d L1 %.3e (pull %.3e) launches %d"
+                  % (bits, task_edges, gr.V, l1, np.abs(a.astype(np.float64) - ref).sum(),
+                     st.kernel_launches), flush=True)
+            assert l1 <= PR_TOL
+            assert np.abs(a.astype(np.float64) - b).sum() <= PR_TOL
+            assert (b.view(np.uint32) == c.view(np.uint32)).all()  # run to run
+            if task_edges and task_edges < 1000:  # split bins: a finish pass per iteration
+                assert st.kernel_launches == 1 + 3 * 20
+
+
+def test_pagerank_binned_edge_cases(ctx, oracle):
+    # a hub whose out-edges reach every bin, isolated vertices, V not a
+    # multiple of the bin width, iteration counts 0 / 1 / 20, split bins
+    L = 70001
+    V = L + 5
+    rows = [list(range(1, L + 1))] + [[0]] * L + [[], [V - 2], [V - 3], []]
+    off = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint64)
+    col = np.concatenate([np.array(r, np.uint32) for r in rows])
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=True) as g:
+        for bits, te in ((8, 0), (8, 256), (14, 1000)):
+            for it in (0, 1, 20):
+                a, _ = pagerank_with(ctx, g, 2, iters=it)
+                b, _ = pagerank_with(ctx, g, 4, bits, iters=it, task_edges=te)
+                ref = oracle.pagerank(h, 0.85, it)
+                assert np.abs(b.astype(np.float64) - ref).sum() <= PR_TOL, (bits, te, it)
+                assert np.abs(a.astype(np.float64) - b).sum() <= PR_TOL, (bits, te, it)
+                if it == 0:
+                    assert (a.view(np.uint32) == b.view(np.uint32)).all()
+    # one vertex, no edges: the pull handles it
+    with grus.Graph.from_csr(ctx, np.zeros(2, np.uint64), np.zeros(0, np.uint32),
+                             symmetric=True) as g:
+        r, _ = pagerank_with(ctx, g, 4)
+        assert np.allclose(r, [0.15], rtol=1e-6)
+
+
+def test_pagerank_binned_falls_back_to_pull(ctx):
+    # 2^20 vertices in bins of 2^6 is 16384 bins, more than the layout
+    # takes: the plain pull runs and gives its bit-identical ranks
+    with grus.Graph.rmat(ctx, 20) as g:
+        a, sa = pagerank_with(ctx, g, 2, iters=3)
+        b, sb = pagerank_with(ctx, g, 4, 6, iters=3)
+        assert (a.view(np.uint32) == b.view(np.uint32)).all()
+        assert sa.kernel_launches == sb.kernel_launches
 
 
 def sssp_with(ctx, g, src, pull_div):
