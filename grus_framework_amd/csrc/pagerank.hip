@@ -67,10 +67,10 @@ struct PrWork {
   // binned push layout (pr_layout = 4); see the comment above
   // build_binned.  p1 = phase-1 stream; ch_u0 / ch_i0 = phase-1 chunks;
   // dir = per (chunk, bin) slot offset; val = group values (bin-major);
-  // d2 = phase-2 edge stream (u16); gs16 = group index at every 16th edge;
+  // d2 = phase-2 edge stream (u16); gs512 = first group of every slab;
   // task / multi = phase-2 tasks and the bins split over several tasks;
   // gacc = fixed-point sums of split bins (kept zero between iterations)
-  Buf bp_p1, bp_ch_u0, bp_ch_i0, bp_dir, bp_val, bp_d2, bp_gs16, bp_task, bp_multi, bp_gacc;
+  Buf bp_p1, bp_ch_u0, bp_ch_i0, bp_dir, bp_val, bp_d2, bp_gs512, bp_task, bp_multi, bp_gacc;
   uint64_t bp_G = 0, bp_nch = 0, bp_ntask = 0, bp_nmulti = 0;
   uint32_t bp_nbins = 0;
   int bp_bits = 0;
@@ -265,8 +265,11 @@ __device__ __forceinline__ void finish(uint64_t r, float s, const uint64_t *out_
 constexpr int BP_MAX_BITS = 14;       // phase-2 accumulators: 2^14 x u64 = 128 KB of LDS
 constexpr uint32_t BP_MAX_BINS = 8192;  // phase-1 directory row in LDS: <= 32 KB
 constexpr int BP_BLOCK = 1024;        // phase-2 workgroup (one per CU: its LDS)
-constexpr int BP_EPT = 16;            // phase-2 edges per thread and step (32-byte load)
+constexpr int BP_EPT = 8;             // phase-2 entries per lane and step (16-byte load)
+constexpr int BP_SLAB = WAVE * BP_EPT;  // entries per wave step
 constexpr int BP_FIX = 62;            // sums are kept as floor(x * 2^62) in u64
+constexpr uint32_t BP_SPAN = 8192;    // sources per phase-1 chunk (staged in LDS)
+constexpr int BP_UNROLL = 8;          // loads in flight per thread in the streaming loops
 
 struct BpTask {
   uint32_t e0, e1;  // edge range of the phase-2 stream
@@ -276,14 +279,16 @@ struct BpTask {
 
 // floor(x * 2^BP_FIX) for a finite x in [0, 4): contributions are ranks
 // divided by degrees, and ranks sum to at most 1
+// (branch-free: both shifts are computed and one is selected)
 __device__ __forceinline__ unsigned long long to_fixed(float x) {
   const uint32_t b = __float_as_uint(x);
   const int e = (int)((b >> 23) & 0xFFu);
   const unsigned long long m = (b & 0x7FFFFFu) | (e ? 0x800000u : 0u);
-  int sh = (e ? e : 1) - 150 + BP_FIX;  // x = m * 2^((e ? e : 1) - 150)
-  if (sh > 40) sh = 40;                 // x >= 4 is outside the domain (no UB shift)
-  if (sh >= 0) return m << sh;
-  return sh > -64 ? m >> -sh : 0ull;
+  const int sh = (e ? e : 1) - 150 + BP_FIX;  // x = m * 2^((e ? e : 1) - 150)
+  const int ls = sh < 0 ? 0 : (sh > 40 ? 40 : sh);  // x >= 4 is outside the domain
+  const int rs = sh >= 0 ? 0 : (-sh > 63 ? 63 : -sh);
+  const unsigned long long hi = m << ls, lo = m >> rs;
+  return sh >= 0 ? hi : (sh > -64 ? lo : 0ull);
 }
 
 __device__ __forceinline__ float from_fixed(unsigned long long s) {
@@ -294,74 +299,208 @@ __device__ __forceinline__ float from_fixed(unsigned long long s) {
 // takes the groups of one source chunk in (bin, source) order, so the
 // stores of each bin are one contiguous run; dir[c][bin] turns a stream
 // position into its slot (u32 arithmetic, slots < 2^32).
+// Chunks run in dispatch order, so the chunks in flight write neighbouring
+// runs of every bin (keeping consecutive chunks on one XCD instead measured
+// 6% slower on RMAT s24).
+// The chunk's contributions (at most BP_SPAN sources) are staged in LDS
+// first: read straight from memory, the stream's source order (by bin,
+// then source) made each wave's loads touch dozens of lines (RMAT s24:
+// 0.86 -> 0.51 ms without them).
 __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, const uint32_t *ch_i0,
                                                       const uint32_t *p1, const uint32_t *dir,
                                                       uint32_t nbins, const float *contrib,
                                                       float *val) {
-  extern __shared__ uint32_t sdir[];
+  extern __shared__ uint32_t sdir[];  // nbins directory entries, then BP_SPAN values
+  float *sc = reinterpret_cast<float *>(sdir + nbins);
   const uint64_t c = blockIdx.x;
   const uint32_t *row = dir + c * nbins;
-  for (uint32_t b = threadIdx.x; b < nbins; b += BLOCK) sdir[b] = row[b];
+  const uint32_t u0 = ch_u0[c], span = ch_u0[c + 1] - u0;  // span >= 1
+  // every loop issues BP_UNROLL loads (indices clamped in range) before it
+  // uses any: one memory latency per round instead of one per element
+  for (uint32_t jb = 0; jb < nbins; jb += BLOCK * BP_UNROLL) {
+    uint32_t x[BP_UNROLL];
+#pragma unroll
+    for (int q = 0; q < BP_UNROLL; q++) {
+      const uint32_t j = jb + threadIdx.x + BLOCK * q;
+      x[q] = row[j < nbins ? j : nbins - 1u];
+    }
+#pragma unroll
+    for (int q = 0; q < BP_UNROLL; q++) {
+      const uint32_t j = jb + threadIdx.x + BLOCK * q;
+      if (j < nbins) sdir[j] = x[q];
+    }
+  }
+  for (uint32_t jb = 0; jb < span; jb += BLOCK * BP_UNROLL) {
+    float x[BP_UNROLL];
+#pragma unroll
+    for (int q = 0; q < BP_UNROLL; q++) {
+      const uint32_t j = jb + threadIdx.x + BLOCK * q;
+      x[q] = contrib[u0 + (j < span ? j : span - 1u)];
+    }
+#pragma unroll
+    for (int q = 0; q < BP_UNROLL; q++) {
+      const uint32_t j = jb + threadIdx.x + BLOCK * q;
+      if (j < span) sc[j] = x[q];
+    }
+  }
   __syncthreads();
   const uint32_t i0 = ch_i0[c], i1 = ch_i0[c + 1];
-  const float *cu = contrib + ch_u0[c];
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-    const uint32_t w = __builtin_nontemporal_load(&p1[i]);
-    val[sdir[w >> 16] + i] = cu[w & 0xFFFFu];
+  for (uint32_t ib = i0; ib < i1; ib += BLOCK * BP_UNROLL) {
+    uint32_t w[BP_UNROLL];
+#pragma unroll
+    for (int q = 0; q < BP_UNROLL; q++) {
+      const uint32_t i = ib + threadIdx.x + BLOCK * q;
+      w[q] = __builtin_nontemporal_load(&p1[i < i1 ? i : i1 - 1u]);
+    }
+#pragma unroll
+    for (int q = 0; q < BP_UNROLL; q++) {
+      const uint32_t i = ib + threadIdx.x + BLOCK * q;
+      if (i < i1) val[sdir[w[q] >> 16] + i] = sc[w[q] & 0xFFFFu];
+    }
   }
 }
 
 // Phase 2: one task = a piece of one destination bin's edge stream.  Each
 // stream entry is a u16: bit 15 starts a group, bits 0..14 the destination
-// within the bin.  Threads take 16 entries (one 32-byte load) and the group
-// index of the first from gs16; the values are added into LDS accumulators
-// as fixed-point integers, so the sums do not depend on the order of the
-// atomics.  A bin's only task finishes its rows; pieces of split bins add
-// their sums into gacc, which k_bp_finish completes.
+// within the bin.  A wave step takes one slab of BP_SLAB entries (8 per
+// lane, one 16-byte load).  The group index of every entry follows from
+// the slab's first group (gs512) and a wave prefix count of the flags (one
+// ballot per entry position).  The slab's groups, at most BP_SLAB of them,
+// are loaded into the wave's LDS window with coalesced loads; gathered
+// lane by lane from memory they touched a dozen lines per load (RMAT s24:
+// 0.74 -> 0.48 ms without those loads).  The values are added into the
+// LDS accumulators as fixed-point integers, so the sums do not depend on
+// the order of the atomics.  A bin's only task finishes its rows; pieces of
+// split bins add their sums into gacc, which k_bp_finish completes.
 template <bool kLast>
 __global__ __launch_bounds__(BP_BLOCK) void k_bp_accum(
-    const BpTask *tasks, const uint16_t *d2, const uint32_t *gs16, const float *val, uint64_t V,
+    const BpTask *tasks, const uint16_t *d2, const uint32_t *gs512, const float *val, uint64_t V,
     int bits, const uint64_t *out_off, float *next, unsigned long long *gacc, float base,
     float alpha) {
   __shared__ unsigned long long acc[1 << BP_MAX_BITS];
+  __shared__ float win[BP_BLOCK / WAVE][BP_SLAB];
   const BpTask t = tasks[blockIdx.x];
   const uint64_t v0 = (uint64_t)t.bin << bits;
   const uint32_t nb = (uint32_t)((V - v0) < (1ull << bits) ? (V - v0) : (1ull << bits));
   for (uint32_t i = threadIdx.x; i < nb; i += BP_BLOCK) acc[i] = 0ull;
   __syncthreads();
-  const uint32_t k1 = (uint32_t)((t.e1 + (uint64_t)BP_EPT - 1) / BP_EPT);
-  for (uint32_t k = t.e0 / BP_EPT + threadIdx.x; k < k1; k += BP_BLOCK) {
-    const u32x4 *p = reinterpret_cast<const u32x4 *>(d2 + (uint64_t)k * BP_EPT);
-    const u32x4 a = __builtin_nontemporal_load(p);
-    const u32x4 b = __builtin_nontemporal_load(p + 1);
-    const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    uint32_t g = gs16[k] - 1u;  // group before the chunk's first entry
-    uint32_t gq[BP_EPT], dq[BP_EPT];
-    const uint32_t p0 = k * BP_EPT;
+  constexpr uint32_t NW = BP_BLOCK / WAVE;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  float *wv = win[wid];
+  const uint32_t s1 = (uint32_t)((t.e1 + (uint64_t)BP_SLAB - 1) / BP_SLAB);
+  // software pipeline: the next slab's entries load while this one runs
+  uint32_t s = t.e0 / BP_SLAB + wid;
+  u32x4 nx = {0, 0, 0, 0};
+  uint32_t ngs = 0;
+  if (s < s1) {
+    nx = __builtin_nontemporal_load(
+        reinterpret_cast<const u32x4 *>(d2 + (uint64_t)s * BP_SLAB + lane * BP_EPT));
+    ngs = gs512[s];
+  }
+  for (; s < s1; s += NW) {
+    const u32x4 cur = nx;
+    const uint32_t gbase = ngs;  // groups that start before the slab
+    const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    uint32_t ent[BP_EPT];
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) ent[q] = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
+    uint32_t pre = 0, tot = 0;  // flags in the lanes below; flags in the slab
+    uint64_t b0 = 0;
 #pragma unroll
     for (int q = 0; q < BP_EPT; q++) {
-      const uint32_t e = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
-      g += e >> 15;
-      gq[q] = g;
-      dq[q] = e & 0x7FFFu;
+      const uint64_t bq = __ballot(ent[q] >> 15);
+      if (q == 0) b0 = bq;
+      pre += lane_rank(bq);
+      tot += (uint32_t)__popcll(bq);
     }
+    // window = the slab's groups: the one its first entry continues (when
+    // that entry starts none) and every group started inside it
+    const uint32_t w0 = (b0 & 1ull) ? gbase : gbase - 1u;
+    const uint32_t n = gbase + tot - w0;  // in [1, BP_SLAB]
+    // all loads first (clamped into the window, so no branch splits them),
+    // then the LDS writes
+    float wl[BP_SLAB / WAVE];
+#pragma unroll
+    for (int j = 0; j < BP_SLAB / WAVE; j++) {
+      const uint32_t idx = lane + WAVE * j;
+      wl[j] = val[w0 + (idx < n ? idx : n - 1u)];
+    }
+    // the next slab's entries are requested after the window, so waiting
+    // for the window does not wait for them
+    // (unconditional: past the task's last slab it reloads that slab, which
+    // keeps the wait counts exact)
+    {
+      const uint32_t sn = s + NW < s1 ? s + NW : s1 - 1u;
+      nx = __builtin_nontemporal_load(
+          reinterpret_cast<const u32x4 *>(d2 + (uint64_t)sn * BP_SLAB + lane * BP_EPT));
+      ngs = gs512[sn];
+    }
+#pragma unroll
+    for (int j = 0; j < BP_SLAB / WAVE; j++) wv[lane + WAVE * j] = wl[j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // every entry of the slab has its group inside the window; entries
+    // outside the task (a neighbouring bin's, or padding) add zero, and
+    // their 15-bit destination stays inside the 2^14-entry array
+    uint32_t g = gbase + pre - 1u;  // group before the lane's first entry
+    const uint32_t p0 = s * BP_SLAB + lane * BP_EPT;
     float x[BP_EPT];
 #pragma unroll
     for (int q = 0; q < BP_EPT; q++) {
-      const uint32_t pe = p0 + q;
-      x[q] = (pe >= t.e0 && pe < t.e1) ? val[gq[q]] : 0.0f;
+      g += ent[q] >> 15;
+      x[q] = wv[g - w0];
     }
 #pragma unroll
     for (int q = 0; q < BP_EPT; q++) {
       const uint32_t pe = p0 + q;
-      if (pe >= t.e0 && pe < t.e1) atomicAdd(&acc[dq[q]], to_fixed(x[q]));
+      const unsigned long long keep = 0ull - (unsigned long long)(pe >= t.e0 && pe < t.e1);
+      atomicAdd(&acc[ent[q] & ((1u << BP_MAX_BITS) - 1u)], to_fixed(x[q]) & keep);
     }
+    // the window is rewritten by the next slab only after these reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nb; i += BP_BLOCK) {
-    const unsigned long long s = acc[i];
-    if (t.single) finish<kLast>(v0 + i, from_fixed(s), out_off, next, base, alpha);
-    else if (s) atomicAdd(&gacc[v0 + i], s);
+  constexpr int R = (1 << BP_MAX_BITS) / BP_BLOCK;  // rows per thread at most
+  if (!t.single) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const uint32_t i = threadIdx.x + BP_BLOCK * r;
+      if (i < nb) {
+        const unsigned long long sum = acc[i];
+        if (sum) atomicAdd(&gacc[v0 + i], sum);
+      }
+    }
+  } else if (kLast) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const uint32_t i = threadIdx.x + BP_BLOCK * r;
+      if (i < nb) next[v0 + i] = finish_row(from_fixed(acc[i]), base, alpha);
+    }
+  } else {
+    // degrees of all the thread's rows are requested before any is used
+#pragma unroll
+    for (int rb = 0; rb < R; rb += BP_UNROLL) {
+      uint64_t a[BP_UNROLL], b[BP_UNROLL];
+#pragma unroll
+      for (int q = 0; q < BP_UNROLL; q++) {
+        const uint32_t i = threadIdx.x + BP_BLOCK * (rb + q);
+        const uint64_t v = v0 + (i < nb ? i : nb - 1u);
+        a[q] = out_off[v];
+        b[q] = out_off[v + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < BP_UNROLL; q++) {
+        const uint32_t i = threadIdx.x + BP_BLOCK * (rb + q);
+        if (i < nb) {
+          const float rank = finish_row(from_fixed(acc[i]), base, alpha);
+          const uint64_t d = b[q] - a[q];
+          next[v0 + i] = d ? rank / (float)d : 0.0f;
+        }
+      }
+    }
   }
 }
 
@@ -456,7 +595,7 @@ __global__ void k_bp_d2_keys(const uint64_t *off, const uint32_t *col, uint64_t 
     }
 }
 
-// d2 (u16, padded with zeros to a multiple of BP_EPT), its group flags as
+// d2 (u16, padded with zeros to a multiple of BP_SLAB), its group flags as
 // u32 for the scan, and the first edge of every bin
 __global__ void k_bp_d2(const uint64_t *key, const uint32_t *ent, uint64_t E, uint64_t Epad,
                         uint32_t nbins, uint16_t *d2, uint32_t *f, uint32_t *bstart) {
@@ -477,10 +616,11 @@ __global__ void k_bp_d2(const uint64_t *key, const uint32_t *ent, uint64_t E, ui
   }
 }
 
-__global__ void k_bp_gs16(const uint32_t *excl, uint64_t E, uint64_t nk, uint32_t *gs16) {
+// gs512[s] = groups that start before slab s
+__global__ void k_bp_gs512(const uint32_t *excl, uint64_t E, uint64_t ns, uint32_t *gs512) {
   uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; k < nk; k += stride) gs16[k] = excl[k * BP_EPT < E ? k * BP_EPT : E - 1];
+  for (; k < ns; k += stride) gs512[k] = excl[k * BP_SLAB < E ? k * BP_SLAB : E - 1];
 }
 
 // PageRank as a window reduction: sum of contrib[u] over the in-edges
@@ -657,7 +797,7 @@ bool build_blocked(Graph &g, PrWork &w, int bits) {
 // share a bin.  All edges of a group carry the same value contrib[u].
 //   phase 1  writes one value per group into val[], bin-major and by
 //            source inside a bin.  Sources are cut into chunks of at most
-//            32K groups and 64K sources; a chunk's groups are streamed in
+//            32K groups and 8K sources; a chunk's groups are streamed in
 //            (bin, source) order (p1: bin << 16 | source - first source),
 //            so each bin receives one contiguous run per chunk.
 //   phase 2  streams each bin's edges in (source, destination) order as
@@ -671,7 +811,6 @@ bool build_blocked(Graph &g, PrWork &w, int bits) {
 // the graph does not fit the layout or the device lacks memory for the
 // build (peak ~ 40E + 16V bytes).
 constexpr uint32_t BP_CHUNK_GROUPS = 32768;
-constexpr uint64_t BP_SPAN = 65536;
 constexpr uint64_t BP_TASK_EDGES = 1ull << 18;  // automatic task size
 
 bool build_binned(Graph &g, PrWork &w, int bits) {
@@ -679,12 +818,12 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
   const CsrView out = g.csr();
   const uint64_t V = g.V, E = g.E;
   const uint64_t nbins = div_up(V, 1ull << bits);
-  if (E == 0 || E >= (1ull << 32) - 64 || V >= (1ull << 32) || nbins > pr::BP_MAX_BINS)
+  if (E == 0 || E >= (1ull << 32) - 1024 || V >= (1ull << 32) || nbins > pr::BP_MAX_BINS)
     return false;
   size_t free_b = 0, total_b = 0;
   GRUS_HIP(hipMemGetInfo(&free_b, &total_b));
   if (free_b < 48 * E + 16 * V + (1ull << 30)) return false;
-  for (Buf *b : {&w.bp_p1, &w.bp_ch_u0, &w.bp_ch_i0, &w.bp_dir, &w.bp_val, &w.bp_d2, &w.bp_gs16,
+  for (Buf *b : {&w.bp_p1, &w.bp_ch_u0, &w.bp_ch_i0, &w.bp_dir, &w.bp_val, &w.bp_d2, &w.bp_gs512,
                  &w.bp_task, &w.bp_multi, &w.bp_gacc})
     b->release();
   w.bp_for = nullptr;
@@ -703,7 +842,13 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
                           ctx.stream));
   ctx.sync();
   const uint64_t G = (uint64_t)last[0] + last[1];
-  // groups per source (goff) -> phase-1 chunks, on the host
+  // groups per source (goff) -> phase-1 chunks, on the host (a source has
+  // at most nbins <= 8192 groups, so it always fits a chunk)
+  static const uint64_t chunk_groups = [] {
+    const char *e = getenv("GRUS_BP_CHUNK_GROUPS");  // experiments only
+    const long v = e ? atol(e) : 0;
+    return v >= (long)pr::BP_MAX_BINS ? (uint64_t)v : (uint64_t)BP_CHUNK_GROUPS;
+  }();
   std::vector<uint32_t> ch_u0, ch_i0;
   {
     Buf goff;
@@ -716,8 +861,8 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
     ctx.sync();
     uint64_t u0 = 0;
     while (u0 < V) {
-      uint64_t u1 = u0 + 1;  // a source has at most nbins <= 8192 groups
-      while (u1 < V && u1 - u0 < BP_SPAN && h[u1 + 1] - h[u0] <= BP_CHUNK_GROUPS) u1++;
+      uint64_t u1 = u0 + 1;
+      while (u1 < V && u1 - u0 < pr::BP_SPAN && h[u1 + 1] - h[u0] <= chunk_groups) u1++;
       ch_u0.push_back((uint32_t)u0);
       ch_i0.push_back((uint32_t)h[u0]);
       u0 = u1;
@@ -725,9 +870,11 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
     ch_i0.push_back((uint32_t)G);
   }
   const uint32_t nch = (uint32_t)ch_u0.size();
-  w.bp_ch_u0.alloc(nch * 4);
+  ch_u0.push_back((uint32_t)V);  // sentinel: the last chunk's end
+  w.bp_ch_u0.alloc((nch + 1) * 4);
   w.bp_ch_i0.alloc((nch + 1) * 4);
-  GRUS_HIP(hipMemcpyAsync(w.bp_ch_u0.p, ch_u0.data(), nch * 4, hipMemcpyHostToDevice, ctx.stream));
+  GRUS_HIP(hipMemcpyAsync(w.bp_ch_u0.p, ch_u0.data(), (nch + 1) * 4, hipMemcpyHostToDevice,
+                          ctx.stream));
   GRUS_HIP(hipMemcpyAsync(w.bp_ch_i0.p, ch_i0.data(), (nch + 1) * 4, hipMemcpyHostToDevice,
                           ctx.stream));
   // group keys (bin << 32 | source), by group id
@@ -781,11 +928,11 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
   gkey.release();
   slot.release();
   // phase-2 stream: CSR edges stably sorted by destination bin
-  const uint64_t Epad = div_up(E, pr::BP_EPT) * pr::BP_EPT;
-  const uint64_t nk = Epad / pr::BP_EPT;
+  const uint64_t Epad = div_up(E, pr::BP_SLAB) * pr::BP_SLAB;
+  const uint64_t nk = Epad / pr::BP_SLAB;
   std::vector<uint32_t> bstart(nbins + 1);
   w.bp_d2.alloc(Epad * 2);
-  w.bp_gs16.alloc(nk * 4);
+  w.bp_gs512.alloc(nk * 4);
   {
     Buf ka, kb, vb, bs;
     ka.alloc(E * 8);
@@ -806,8 +953,8 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
         key, ent, E, Epad, (uint32_t)nbins, w.bp_d2.as<uint16_t>(), f, bs.as<uint32_t>());
     GRUS_LAUNCH_CHECK();
     scan_excl_u32(ctx, f, flag.as<uint32_t>(), E);  // flag now holds the exclusive scan
-    pr::k_bp_gs16<<<grid_cap(ctx, nk, BLOCK), BLOCK, 0, ctx.stream>>>(
-        flag.as<uint32_t>(), E, nk, w.bp_gs16.as<uint32_t>());
+    pr::k_bp_gs512<<<grid_cap(ctx, nk, BLOCK), BLOCK, 0, ctx.stream>>>(
+        flag.as<uint32_t>(), E, nk, w.bp_gs512.as<uint32_t>());
     GRUS_LAUNCH_CHECK();
     GRUS_HIP(hipMemcpyAsync(bstart.data(), bs.p, (nbins + 1) * 4, hipMemcpyDeviceToHost,
                             ctx.stream));
@@ -891,13 +1038,13 @@ void run_binned(Graph &g, PrWork &w, float alpha, int iters, grus_stats *st) {
     const bool last = (it == iters - 1);
     // phase 1 reads contrib; phase 2 overwrites it (or writes the ranks)
     float *next = last ? rank : contrib;
-    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, w.bp_nbins * 4, ctx.stream>>>(
+    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
         w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_p1.as<uint32_t>(),
         w.bp_dir.as<uint32_t>(), w.bp_nbins, contrib, w.bp_val.as<float>());
     GRUS_LAUNCH_CHECK();
     if (last) {
       pr::k_bp_accum<true><<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
-          tasks, w.bp_d2.as<uint16_t>(), w.bp_gs16.as<uint32_t>(), w.bp_val.as<float>(), V, bits,
+          tasks, w.bp_d2.as<uint16_t>(), w.bp_gs512.as<uint32_t>(), w.bp_val.as<float>(), V, bits,
           out.off, next, gacc, base, alpha);
       GRUS_LAUNCH_CHECK();
       if (w.bp_nmulti)
@@ -905,7 +1052,7 @@ void run_binned(Graph &g, PrWork &w, float alpha, int iters, grus_stats *st) {
             w.bp_multi.as<uint32_t>(), bits, V, gacc, out.off, next, base, alpha);
     } else {
       pr::k_bp_accum<false><<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
-          tasks, w.bp_d2.as<uint16_t>(), w.bp_gs16.as<uint32_t>(), w.bp_val.as<float>(), V, bits,
+          tasks, w.bp_d2.as<uint16_t>(), w.bp_gs512.as<uint32_t>(), w.bp_val.as<float>(), V, bits,
           out.off, next, gacc, base, alpha);
       GRUS_LAUNCH_CHECK();
       if (w.bp_nmulti)
