@@ -277,18 +277,17 @@ struct BpTask {
   uint32_t single;  // 1: the bin's only task, which finishes its rows
 };
 
-// floor(x * 2^BP_FIX) for a finite x in [0, 4): contributions are ranks
-// divided by degrees, and ranks sum to at most 1
-// (branch-free: both shifts are computed and one is selected)
+// floor(x * 2^62) for a finite x in [0, 4): contributions are ranks
+// divided by degrees, and ranks sum to at most 1.  With y = x * 2^30
+// (exact), the high word is trunc(y) and the low word trunc(fract(y) *
+// 2^32); fract(y) is exact and has at most 24 significant bits, so both
+// conversions are exact.
 __device__ __forceinline__ unsigned long long to_fixed(float x) {
-  const uint32_t b = __float_as_uint(x);
-  const int e = (int)((b >> 23) & 0xFFu);
-  const unsigned long long m = (b & 0x7FFFFFu) | (e ? 0x800000u : 0u);
-  const int sh = (e ? e : 1) - 150 + BP_FIX;  // x = m * 2^((e ? e : 1) - 150)
-  const int ls = sh < 0 ? 0 : (sh > 40 ? 40 : sh);  // x >= 4 is outside the domain
-  const int rs = sh >= 0 ? 0 : (-sh > 63 ? 63 : -sh);
-  const unsigned long long hi = m << ls, lo = m >> rs;
-  return sh >= 0 ? hi : (sh > -64 ? lo : 0ull);
+  static_assert(BP_FIX == 62, "the split below assumes 2^62");
+  const float y = x * 0x1p30f;
+  const uint32_t hi = (uint32_t)y;
+  const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(y) * 0x1p32f);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
 __device__ __forceinline__ float from_fixed(unsigned long long s) {
@@ -363,9 +362,10 @@ __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, con
 // Phase 2: one task = a piece of one destination bin's edge stream.  Each
 // stream entry is a u16: bit 15 starts a group, bits 0..14 the destination
 // within the bin.  A wave step takes one slab of BP_SLAB entries (8 per
-// lane, one 16-byte load).  The group index of every entry follows from
+// lane, one 16-byte load; slabs are stored transposed, so lane l gets the
+// entries l, l + 64, ...).  The group index of every entry follows from
 // the slab's first group (gs512) and a wave prefix count of the flags (one
-// ballot per entry position).  The slab's groups, at most BP_SLAB of them,
+// ballot per row of 64 entries).  The slab's groups, at most BP_SLAB of them,
 // are loaded into the wave's LDS window with coalesced loads; gathered
 // lane by lane from memory they touched a dozen lines per load (RMAT s24:
 // 0.74 -> 0.48 ms without those loads).  The values are added into the
@@ -401,34 +401,34 @@ __global__ __launch_bounds__(BP_BLOCK) void k_bp_accum(
     const u32x4 cur = nx;
     const uint32_t gbase = ngs;  // groups that start before the slab
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    // lane l holds the slab's entries 64 q + l, q = 0..7 (transposed layout)
     uint32_t ent[BP_EPT];
 #pragma unroll
     for (int q = 0; q < BP_EPT; q++) ent[q] = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
-    uint32_t pre = 0, tot = 0;  // flags in the lanes below; flags in the slab
+    // group of entry 64 q + l: gbase - 1 + (flags of the rows before q) +
+    // (flags of row q up to lane l); row totals are wave-uniform
+    uint32_t gq[BP_EPT];
+    uint32_t run = gbase - 1u;
     uint64_t b0 = 0;
 #pragma unroll
     for (int q = 0; q < BP_EPT; q++) {
       const uint64_t bq = __ballot(ent[q] >> 15);
       if (q == 0) b0 = bq;
-      pre += lane_rank(bq);
-      tot += (uint32_t)__popcll(bq);
+      gq[q] = run + lane_rank(bq) + (ent[q] >> 15);
+      run += (uint32_t)__popcll(bq);
     }
     // window = the slab's groups: the one its first entry continues (when
     // that entry starts none) and every group started inside it
     const uint32_t w0 = (b0 & 1ull) ? gbase : gbase - 1u;
-    const uint32_t n = gbase + tot - w0;  // in [1, BP_SLAB]
-    // all loads first (clamped into the window, so no branch splits them),
-    // then the LDS writes
+    // at most BP_SLAB groups from w0 on; val is padded by BP_SLAB, so the
+    // window loads need no bound
     float wl[BP_SLAB / WAVE];
 #pragma unroll
-    for (int j = 0; j < BP_SLAB / WAVE; j++) {
-      const uint32_t idx = lane + WAVE * j;
-      wl[j] = val[w0 + (idx < n ? idx : n - 1u)];
-    }
+    for (int j = 0; j < BP_SLAB / WAVE; j++) wl[j] = val[w0 + lane + WAVE * j];
     // the next slab's entries are requested after the window, so waiting
-    // for the window does not wait for them
-    // (unconditional: past the task's last slab it reloads that slab, which
-    // keeps the wait counts exact)
+    // for the window does not wait for them (unconditional: past the
+    // task's last slab it reloads that slab, which keeps the wait counts
+    // exact)
     {
       const uint32_t sn = s + NW < s1 ? s + NW : s1 - 1u;
       nx = __builtin_nontemporal_load(
@@ -440,22 +440,23 @@ __global__ __launch_bounds__(BP_BLOCK) void k_bp_accum(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // every entry of the slab has its group inside the window; entries
-    // outside the task (a neighbouring bin's, or padding) add zero, and
-    // their 15-bit destination stays inside the 2^14-entry array
-    uint32_t g = gbase + pre - 1u;  // group before the lane's first entry
-    const uint32_t p0 = s * BP_SLAB + lane * BP_EPT;
     float x[BP_EPT];
 #pragma unroll
-    for (int q = 0; q < BP_EPT; q++) {
-      g += ent[q] >> 15;
-      x[q] = wv[g - w0];
-    }
+    for (int q = 0; q < BP_EPT; q++) x[q] = wv[gq[q] - w0];  // neighbouring lanes: near addresses
+    const uint32_t sb = s * BP_SLAB;
+    if (sb >= t.e0 && sb + BP_SLAB <= t.e1) {  // wave-uniform: the whole slab is the task's
 #pragma unroll
-    for (int q = 0; q < BP_EPT; q++) {
-      const uint32_t pe = p0 + q;
-      const unsigned long long keep = 0ull - (unsigned long long)(pe >= t.e0 && pe < t.e1);
-      atomicAdd(&acc[ent[q] & ((1u << BP_MAX_BITS) - 1u)], to_fixed(x[q]) & keep);
+      for (int q = 0; q < BP_EPT; q++)
+        atomicAdd(&acc[ent[q] & ((1u << BP_MAX_BITS) - 1u)], to_fixed(x[q]));
+    } else {
+      // a slab shared with a neighbouring bin (or padding): entries outside
+      // the task add zero; their destination bits stay inside the array
+#pragma unroll
+      for (int q = 0; q < BP_EPT; q++) {
+        const uint32_t pe = sb + WAVE * q + lane;
+        const unsigned long long keep = 0ull - (unsigned long long)(pe >= t.e0 && pe < t.e1);
+        atomicAdd(&acc[ent[q] & ((1u << BP_MAX_BITS) - 1u)], to_fixed(x[q]) & keep);
+      }
     }
     // the window is rewritten by the next slab only after these reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -595,18 +596,24 @@ __global__ void k_bp_d2_keys(const uint64_t *off, const uint32_t *col, uint64_t 
     }
 }
 
-// d2 (u16, padded with zeros to a multiple of BP_SLAB), its group flags as
+// d2 (u16, padded with zeros to a multiple of BP_SLAB, each slab stored
+// transposed), its group flags as
 // u32 for the scan, and the first edge of every bin
 __global__ void k_bp_d2(const uint64_t *key, const uint32_t *ent, uint64_t E, uint64_t Epad,
                         uint32_t nbins, uint16_t *d2, uint32_t *f, uint32_t *bstart) {
   uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; p < Epad; p += stride) {
+    // slab-transposed: logical entry r = 64 q + l of a slab is stored in
+    // lane l's 16 bytes, slot q, so one 16-byte load per lane gives each
+    // lane the entries l, l + 64, ..., l + 448
+    const uint64_t r = p % BP_SLAB;
+    const uint64_t phys = p - r + (r % WAVE) * BP_EPT + r / WAVE;
     if (p >= E) {
-      d2[p] = 0;
+      d2[phys] = 0;
       continue;
     }
-    d2[p] = (uint16_t)ent[p];
+    d2[phys] = (uint16_t)ent[p];
     f[p] = ent[p] >> 15;
     const uint64_t kb = key[p];
     const uint64_t ka = p ? key[p - 1] + 1 : 0;  // bins (key[p-1], key[p]] start here
@@ -962,7 +969,8 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
   }
   flag.release();
   excl.release();
-  w.bp_val.alloc(G * 4);
+  w.bp_val.alloc((G + pr::BP_SLAB) * 4);  // a window may read up to BP_SLAB past the last group
+  GRUS_HIP(hipMemsetAsync(w.bp_val.p, 0, (G + pr::BP_SLAB) * 4, ctx.stream));
   ctx.sync();  // host vectors above are copied before they go out of scope
   w.bp_bstart = std::move(bstart);
   w.bp_task_edges = 0;
