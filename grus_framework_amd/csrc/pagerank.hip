@@ -65,12 +65,15 @@ struct PrWork {
   int pb_bits = 0;
   const void *pb_for = nullptr;  // offsets the layout was built on
   // binned push layout (pr_layout = 4); see the comment above
-  // build_binned.  p1 = phase-1 stream; ch_u0 / ch_i0 = phase-1 chunks;
-  // dir = per (chunk, bin) slot offset; val = group values (bin-major);
+  // build_binned.  p1 = phase-1 stream (u16); ch_u0 / ch_i0 / ch_r0 =
+  // first source / stream position / run of every phase-1 chunk; rtab =
+  // slot offset of every run; rbase = runs started before every slab;
+  // val = group values (bin-major);
   // d2 = phase-2 edge stream (u16); gs512 = first group of every slab;
   // task / multi = phase-2 tasks and the bins split over several tasks;
   // gacc = fixed-point sums of split bins (kept zero between iterations)
-  Buf bp_p1, bp_ch_u0, bp_ch_i0, bp_dir, bp_val, bp_d2, bp_gs512, bp_task, bp_multi, bp_gacc;
+  Buf bp_p1, bp_ch_u0, bp_ch_i0, bp_ch_r0, bp_rtab, bp_rbase, bp_val, bp_d2, bp_gs512, bp_task,
+      bp_multi, bp_gacc;
   uint64_t bp_G = 0, bp_nch = 0, bp_ntask = 0, bp_nmulti = 0;
   uint32_t bp_nbins = 0;
   int bp_bits = 0;
@@ -296,37 +299,42 @@ __device__ __forceinline__ float from_fixed(unsigned long long s) {
 
 // Phase 1: val[slot(j)] = contrib[src(j)] for every group j.  Workgroup c
 // takes the groups of one source chunk in (bin, source) order, so the
-// stores of each bin are one contiguous run; dir[c][bin] turns a stream
-// position into its slot (u32 arithmetic, slots < 2^32).
-// Chunks run in dispatch order, so the chunks in flight write neighbouring
-// runs of every bin (keeping consecutive chunks on one XCD instead measured
-// 6% slower on RMAT s24).
-// The chunk's contributions (at most BP_SPAN sources) are staged in LDS
-// first: read straight from memory, the stream's source order (by bin,
-// then source) made each wave's loads touch dozens of lines (RMAT s24:
-// 0.86 -> 0.51 ms without them).
+// stores of each bin are one contiguous run.  The stream holds one u16 per
+// group: bit 15 starts a run, bits 0..12 the source within the chunk.
+// Slabs of BP_SLAB entries are stored transposed (lane l of a wave gets
+// the entries l, l + 64, ... with one 16-byte load), so each row of 64
+// stores is consecutive within a run.  The run of an entry is rbase[slab]
+// plus a wave prefix count of the run flags; rtab[run] turns a stream
+// position into its slot (u32 arithmetic, slots < 2^32).  Chunks run in
+// dispatch order, so the chunks in flight write neighbouring runs of every
+// bin (keeping consecutive chunks on one XCD instead measured 6% slower on
+// RMAT s24).  The chunk's contributions (at most BP_SPAN sources) and run
+// offsets are staged in LDS first: read straight from memory, the stream's
+// source order made each wave's loads touch dozens of lines.
 __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, const uint32_t *ch_i0,
-                                                      const uint32_t *p1, const uint32_t *dir,
+                                                      const uint32_t *ch_r0, const uint16_t *p1,
+                                                      const uint32_t *rbase, const uint32_t *rtab,
                                                       uint32_t nbins, const float *contrib,
                                                       float *val) {
-  extern __shared__ uint32_t sdir[];  // nbins directory entries, then BP_SPAN values
-  float *sc = reinterpret_cast<float *>(sdir + nbins);
+  extern __shared__ uint32_t sr[];  // the chunk's run offsets (<= nbins), then BP_SPAN values
+  float *sc = reinterpret_cast<float *>(sr + nbins);
   const uint64_t c = blockIdx.x;
-  const uint32_t *row = dir + c * nbins;
   const uint32_t u0 = ch_u0[c], span = ch_u0[c + 1] - u0;  // span >= 1
-  // every loop issues BP_UNROLL loads (indices clamped in range) before it
-  // uses any: one memory latency per round instead of one per element
-  for (uint32_t jb = 0; jb < nbins; jb += BLOCK * BP_UNROLL) {
+  const uint32_t r0 = ch_r0[c], nr = ch_r0[c + 1] - r0;    // nr <= nbins
+  // every staging loop issues BP_UNROLL loads (indices clamped in range)
+  // before it uses any: one memory latency per round instead of one per
+  // element
+  for (uint32_t jb = 0; jb < nr; jb += BLOCK * BP_UNROLL) {
     uint32_t x[BP_UNROLL];
 #pragma unroll
     for (int q = 0; q < BP_UNROLL; q++) {
       const uint32_t j = jb + threadIdx.x + BLOCK * q;
-      x[q] = row[j < nbins ? j : nbins - 1u];
+      x[q] = rtab[r0 + (j < nr ? j : nr - 1u)];
     }
 #pragma unroll
     for (int q = 0; q < BP_UNROLL; q++) {
       const uint32_t j = jb + threadIdx.x + BLOCK * q;
-      if (j < nbins) sdir[j] = x[q];
+      if (j < nr) sr[j] = x[q];
     }
   }
   for (uint32_t jb = 0; jb < span; jb += BLOCK * BP_UNROLL) {
@@ -344,17 +352,49 @@ __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, con
   }
   __syncthreads();
   const uint32_t i0 = ch_i0[c], i1 = ch_i0[c + 1];
-  for (uint32_t ib = i0; ib < i1; ib += BLOCK * BP_UNROLL) {
-    uint32_t w[BP_UNROLL];
-#pragma unroll
-    for (int q = 0; q < BP_UNROLL; q++) {
-      const uint32_t i = ib + threadIdx.x + BLOCK * q;
-      w[q] = __builtin_nontemporal_load(&p1[i < i1 ? i : i1 - 1u]);
+  if (i0 == i1) return;  // a chunk of sources without out-edges
+  constexpr uint32_t NW = BLOCK / WAVE;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const uint32_t s1 = (i1 + BP_SLAB - 1u) / BP_SLAB;
+  uint32_t s = i0 / BP_SLAB + wid;
+  u32x4 nx = {0, 0, 0, 0};
+  uint32_t nrb = 0;
+  if (s < s1) {
+    nx = __builtin_nontemporal_load(
+        reinterpret_cast<const u32x4 *>(p1 + (uint64_t)s * BP_SLAB + lane * BP_EPT));
+    nrb = rbase[s];
+  }
+  for (; s < s1; s += NW) {
+    const u32x4 cur = nx;
+    const uint32_t rb = nrb;  // runs started before the slab
+    if (s + NW < s1) {  // the next slab loads while this one is stored
+      nx = __builtin_nontemporal_load(
+          reinterpret_cast<const u32x4 *>(p1 + (uint64_t)(s + NW) * BP_SLAB + lane * BP_EPT));
+      nrb = rbase[s + NW];
     }
+    const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    uint32_t ent[BP_EPT], rq[BP_EPT];
+    uint32_t run = rb - 1u;
 #pragma unroll
-    for (int q = 0; q < BP_UNROLL; q++) {
-      const uint32_t i = ib + threadIdx.x + BLOCK * q;
-      if (i < i1) val[sdir[w[q] >> 16] + i] = sc[w[q] & 0xFFFFu];
+    for (int q = 0; q < BP_EPT; q++) {
+      ent[q] = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
+      const uint64_t bq = __ballot(ent[q] >> 15);
+      rq[q] = run + lane_rank(bq) + (ent[q] >> 15);
+      run += (uint32_t)__popcll(bq);
+    }
+    const uint32_t sb = s * BP_SLAB;
+    if (sb >= i0 && sb + BP_SLAB <= i1) {  // wave-uniform: the whole slab is the chunk's
+#pragma unroll
+      for (int q = 0; q < BP_EPT; q++) {
+        const uint32_t i = sb + WAVE * q + lane;
+        val[sr[rq[q] - r0] + i] = sc[ent[q] & 0x1FFFu];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < BP_EPT; q++) {
+        const uint32_t i = sb + WAVE * q + lane;
+        if (i >= i0 && i < i1) val[sr[rq[q] - r0] + i] = sc[ent[q] & 0x1FFFu];
+      }
     }
   }
 }
@@ -558,15 +598,22 @@ __global__ void k_bp_slots(const uint32_t *sorted, uint64_t G, uint32_t *slot) {
   for (; s < G; s += stride) slot[sorted[s]] = (uint32_t)s;
 }
 
-// phase-1 stream entry and directory: position i of chunk c holds group
-// sorted[i]; the first position of each (chunk, bin) run records
-// slot - position
+// phase-1 stream entry and run data: position i of chunk c holds group
+// sorted[i]; a run starts at the chunk's first position and wherever the
+// bin changes, and records slot - position.  The u16 entries are written
+// slab-transposed (as the phase-2 stream).
 __global__ void k_bp_p1(const uint32_t *sorted, const uint64_t *gkey, const uint32_t *slot,
-                        uint64_t G, const uint32_t *ch_u0, const uint32_t *ch_i0, uint32_t nch,
-                        uint32_t nbins, uint32_t *p1, uint32_t *dir) {
+                        uint64_t G, uint64_t Gpad, const uint32_t *ch_u0, const uint32_t *ch_i0,
+                        uint32_t nch, uint16_t *p1, uint32_t *rflag, uint32_t *rdel) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < G; i += stride) {
+  for (; i < Gpad; i += stride) {
+    const uint64_t r = i % BP_SLAB;
+    const uint64_t phys = i - r + (r % WAVE) * BP_EPT + r / WAVE;
+    if (i >= G) {
+      p1[phys] = 0;
+      continue;
+    }
     uint32_t lo = 0, hi = nch;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -576,10 +623,25 @@ __global__ void k_bp_p1(const uint32_t *sorted, const uint64_t *gkey, const uint
     const uint32_t j = sorted[i];
     const uint32_t bin = (uint32_t)(gkey[j] >> 32);
     const uint32_t src = (uint32_t)gkey[j];
-    p1[i] = (bin << 16) | (src - ch_u0[lo]);
     const bool run_start = (i == ch_i0[lo]) || (uint32_t)(gkey[sorted[i - 1]] >> 32) != bin;
-    if (run_start) dir[(uint64_t)lo * nbins + bin] = slot[j] - (uint32_t)i;
+    p1[phys] = (uint16_t)((run_start ? 0x8000u : 0u) | (src - ch_u0[lo]));
+    rflag[i] = run_start ? 1u : 0u;
+    rdel[i] = run_start ? slot[j] - (uint32_t)i : 0u;
   }
+}
+
+// run table, runs before every slab and first run of every chunk
+__global__ void k_bp_runs(const uint32_t *rflag, const uint32_t *rexcl, const uint32_t *rdel,
+                          uint64_t G, uint64_t nslab, const uint32_t *ch_i0, uint32_t nch,
+                          uint32_t R, uint32_t *rtab, uint32_t *rbase, uint32_t *ch_r0) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < G; i += stride) {
+    if (rflag[i]) rtab[rexcl[i]] = rdel[i];
+    if (i % BP_SLAB == 0 && i / BP_SLAB < nslab) rbase[i / BP_SLAB] = rexcl[i];
+  }
+  i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  for (; i <= nch; i += stride) ch_r0[i] = ch_i0[i] < G ? rexcl[ch_i0[i]] : R;
 }
 
 // phase-2 stream keys: CSR edge e -> (bin of its destination, entry)
@@ -830,8 +892,8 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
   size_t free_b = 0, total_b = 0;
   GRUS_HIP(hipMemGetInfo(&free_b, &total_b));
   if (free_b < 48 * E + 16 * V + (1ull << 30)) return false;
-  for (Buf *b : {&w.bp_p1, &w.bp_ch_u0, &w.bp_ch_i0, &w.bp_dir, &w.bp_val, &w.bp_d2, &w.bp_gs512,
-                 &w.bp_task, &w.bp_multi, &w.bp_gacc})
+  for (Buf *b : {&w.bp_p1, &w.bp_ch_u0, &w.bp_ch_i0, &w.bp_ch_r0, &w.bp_rtab, &w.bp_rbase,
+                 &w.bp_val, &w.bp_d2, &w.bp_gs512, &w.bp_task, &w.bp_multi, &w.bp_gacc})
     b->release();
   w.bp_for = nullptr;
   const unsigned rows_grid = grid_cap(ctx, V * WAVE, BLOCK);
@@ -909,9 +971,11 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
     GRUS_LAUNCH_CHECK();
     ctx.sync();
   }
-  w.bp_p1.alloc(G * 4);
-  w.bp_dir.alloc((uint64_t)nch * nbins * 4);
-  GRUS_HIP(hipMemsetAsync(w.bp_dir.p, 0, (uint64_t)nch * nbins * 4, ctx.stream));
+  const uint64_t Gpad = div_up(G, pr::BP_SLAB) * pr::BP_SLAB;
+  const uint64_t nslab1 = Gpad / pr::BP_SLAB;
+  w.bp_p1.alloc(Gpad * 2);
+  w.bp_rbase.alloc(nslab1 * 4);
+  w.bp_ch_r0.alloc((nch + 1) * 4);
   {  // phase-1 stream: (chunk, bin, source) order
     Buf ka, kb, va, vb;
     ka.alloc(G * 8);
@@ -925,10 +989,25 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
     uint32_t *sorted = nullptr;
     sort_pairs_u64_u32(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<uint32_t>(),
                        vb.as<uint32_t>(), G, &sorted, 16 + bits_for(nch));
-    pr::k_bp_p1<<<grid_cap(ctx, G, BLOCK), BLOCK, 0, ctx.stream>>>(
-        sorted, gkey.as<uint64_t>(), slot.as<uint32_t>(), G, w.bp_ch_u0.as<uint32_t>(),
-        w.bp_ch_i0.as<uint32_t>(), nch, (uint32_t)nbins, w.bp_p1.as<uint32_t>(),
-        w.bp_dir.as<uint32_t>());
+    // run flags, offsets and their scan reuse buffers the sort is done with:
+    // the other value buffer, and ka (2G u32)
+    uint32_t *rflag = sorted == va.as<uint32_t>() ? vb.as<uint32_t>() : va.as<uint32_t>();
+    uint32_t *rdel = ka.as<uint32_t>();
+    uint32_t *rexcl = rdel + G;
+    pr::k_bp_p1<<<grid_cap(ctx, Gpad, BLOCK), BLOCK, 0, ctx.stream>>>(
+        sorted, gkey.as<uint64_t>(), slot.as<uint32_t>(), G, Gpad, w.bp_ch_u0.as<uint32_t>(),
+        w.bp_ch_i0.as<uint32_t>(), nch, w.bp_p1.as<uint16_t>(), rflag, rdel);
+    GRUS_LAUNCH_CHECK();
+    scan_excl_u32(ctx, rflag, rexcl, G);
+    uint32_t lastr[2];
+    GRUS_HIP(hipMemcpyAsync(&lastr[0], rexcl + G - 1, 4, hipMemcpyDeviceToHost, ctx.stream));
+    GRUS_HIP(hipMemcpyAsync(&lastr[1], rflag + G - 1, 4, hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    const uint32_t R = lastr[0] + lastr[1];
+    w.bp_rtab.alloc((uint64_t)std::max<uint32_t>(R, 1) * 4);
+    pr::k_bp_runs<<<grid_cap(ctx, G, BLOCK), BLOCK, 0, ctx.stream>>>(
+        rflag, rexcl, rdel, G, nslab1, w.bp_ch_i0.as<uint32_t>(), nch, R,
+        w.bp_rtab.as<uint32_t>(), w.bp_rbase.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>());
     GRUS_LAUNCH_CHECK();
     ctx.sync();
   }
@@ -1047,8 +1126,9 @@ void run_binned(Graph &g, PrWork &w, float alpha, int iters, grus_stats *st) {
     // phase 1 reads contrib; phase 2 overwrites it (or writes the ranks)
     float *next = last ? rank : contrib;
     pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
-        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_p1.as<uint32_t>(),
-        w.bp_dir.as<uint32_t>(), w.bp_nbins, contrib, w.bp_val.as<float>());
+        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
+        w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
+        contrib, w.bp_val.as<float>());
     GRUS_LAUNCH_CHECK();
     if (last) {
       pr::k_bp_accum<true><<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
