@@ -13,7 +13,7 @@ component) / wall time of the timed loop (each call returns when the device
 is idle).
 
 Also reported (the rest of the metric): SSSP GTEPS on the 4096x4096 grid
-and on weighted RMAT s24, PageRank pull GB/s and CC on RMAT s24, batched
+and on weighted RMAT s24, PageRank GB/s and CC on RMAT s24, batched
 serving on RMAT s22 (queries/s, p50/p99, vs a one-query-at-a-time loop) and
 capped-HBM placement on RMAT s27 -- each next to an OpenMP CPU baseline or
 the resident run where one applies.  One JSON line on stdout.
@@ -283,7 +283,8 @@ def run_extra(grus, O, ctx, g, h, threads, args):
 
 
 def run_pagerank(grus, O, g, h, threads, args, out):
-    """PageRank pull, 20 iterations, FP32, on the headline RMAT graph."""
+    """PageRank, 20 iterations, FP32 ranks, on the headline RMAT graph (the
+    library's automatic layout: binned push on HBM-resident graphs)."""
     grus.pagerank(g, copy=False)
     reps = 3
     ms = 0.0
@@ -297,13 +298,16 @@ def run_pagerank(grus, O, g, h, threads, args, out):
     B = 20 * per_iter
     pr = {"ms_20_iters": round(ms, 3), "GBps_algorithmic": round(B / ms / 1e6, 1),
           "frac_hbm_peak": round(B / ms / 1e6 / HBM_PEAK_GBS, 4)}
-    pr["roofline"] = {"kernel": "pr::k_pull_window (one iteration per launch)", "bound": "hbm",
+    pr["roofline"] = {"kernel": "one PageRank iteration, binned push layout "
+                                "(pr::k_bp_scatter + pr::k_bp_accum + pr::k_bp_finish)",
+                      "bound": "hbm",
                       "achieved": round(per_iter / (ms / 20 * 1e-3) / 1e9, 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(per_iter / (ms / 20 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                       "traffic": traffic("pagerank_rmat_s24_iteration"),
                       "algorithmic_bytes": int(per_iter),
-                      "note": "launch time includes the small fixup kernel"}
+                      "note": "device time of the 20-iteration call / 20 (one init "
+                              "launch included); the unit is the iteration"}
     if not args.no_cpu:
         _, cms = O.bl_pagerank(h, 0.85, 2, threads)  # 2-iteration CPU sample, scaled
         pr["cpu_GBps"] = round((B / 10) / (cms * 1e-3) / 1e9, 2)

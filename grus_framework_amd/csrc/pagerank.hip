@@ -1,4 +1,4 @@
-// pagerank.hip -- PageRank, pull direction, FP32, fixed iteration count.
+// pagerank.hip -- PageRank, FP32 ranks, fixed iteration count.
 //
 // Reference: samples/pagerank/pr.cu (pr_init :17-33, pr_dd0 :34-43,
 // updater :78-90, pull driver :134-168).  The reference runs an
@@ -12,7 +12,21 @@
 // by the CSC degree in its pull path, :146, which only agrees on symmetric
 // graphs).  Dangling mass is dropped, as in the reference.
 //
-// Kernel (SpMV over the in-edge CSR): the edge array is cut into windows of
+// Layouts (grus_tuning::pr_layout): the binned push (4, automatic on HBM
+// graphs that fit it), the plain pull (2, everything else), propagation
+// blocking over the pull (3) and a degree-sorted relabelled pull (1).
+//
+// Binned push (pr_layout = 4; build_binned, k_bp_scatter, k_bp_accum):
+// destinations are cut into bins of 2^14 vertices and the out-edges of each
+// source into groups whose destinations share a bin.  Phase 1 writes one
+// value per group into per-bin runs; phase 2 streams each bin's 16-bit edge
+// entries and group values and adds them into LDS accumulators as 64-bit
+// fixed point (floor(x * 2^62)).  Integer sums are exact and order-free, so
+// the ranks are bit-identical run to run and whatever the bin width or task
+// split, and closer to the FP64 oracle than an FP32 sum.  Every global
+// access is a sequential stream; the random accesses hit LDS.
+//
+// Plain pull (SpMV over the in-edge CSR): the edge array is cut into windows of
 // WIN edges, one per workgroup, WIN_EPT consecutive edges per thread.
 // Each thread gathers contrib[col[e]] for its edges into registers and the
 // window is reduced by row segments (merge-style): rows that start and end

@@ -8,9 +8,14 @@
 //         (pull->push: n_f < V/beta) and the bottom-up lane/wave cut-off
 //         (edges a lane scans before handing a vertex to a wave)
 //   SSSP  bucket width delta, and the frontier size at which a round pulls
-//   PR    edge layout (blocked / plain pull) and the blocked bin width
-// Every candidate computes the same result (the settings only change the
-// schedule), so tuning never changes answers.  The winner is written into
+//   PR    edge layout (binned push / blocked / plain pull), the bin width
+//         and the binned layout's task size
+// Every BFS / SSSP candidate computes the same result (the settings only
+// change the schedule), so tuning never changes those answers.  PageRank
+// layouts sum in different orders (ranks agree to float rounding); within
+// the binned layout the bin width and task size leave the ranks bit-
+// identical, since every edge adds the same fixed-point term to the same
+// integer sum.  The winner is written into
 // the context's tuning and returned; grus_graph_signature() gives a key
 // under which callers can cache it.
 #include <algorithm>
@@ -95,9 +100,14 @@ void autotune(Graph &g, int kind, const uint32_t *srcs, int n, grus_tuning *out)
   } else {
     // the relabelled layout is left out: it only pays on graphs whose
     // numbering scatters the hubs, and it costs a full relabelling build
-    sweep<uint32_t>(g, kind, s, &grus_tuning::pr_layout, {2, 3}, &best);
+    sweep<uint32_t>(g, kind, s, &grus_tuning::pr_layout, {2, 3, 4}, &best);
     if (ctx.tuning.pr_layout == 3)
       sweep<uint32_t>(g, kind, s, &grus_tuning::pr_bin_bits, {12, 14, 16, 18}, &best);
+    if (ctx.tuning.pr_layout == 4 || ctx.tuning.pr_layout == 0) {
+      sweep<uint32_t>(g, kind, s, &grus_tuning::pr_bin_bits, {12, 13, 14}, &best);
+      sweep<uint32_t>(g, kind, s, &grus_tuning::pr_task_edges, {1u << 17, 1u << 18, 1u << 19},
+                      &best);
+    }
   }
   if (out) *out = ctx.tuning;
 }

@@ -323,6 +323,16 @@ def test_pagerank_binned_vs_fp64(ctx, oracle, bits, task_edges):
                 assert st.kernel_launches == 1 + 3 * 20
 
 
+def test_pagerank_binned_bits_and_tasks_do_not_change_ranks(ctx):
+    # every edge adds the same fixed-point term to the same integer sum
+    # whatever the bin width and task split: bit-identical ranks
+    with grus.Graph.rmat(ctx, 15) as g:
+        ref, _ = pagerank_with(ctx, g, 4, 14)
+        for bits, te in ((13, 0), (10, 256), (6, 5000), (14, 300), (12, 1 << 18)):
+            r, _ = pagerank_with(ctx, g, 4, bits, task_edges=te)
+            assert (r.view(np.uint32) == ref.view(np.uint32)).all(), (bits, te)
+
+
 def test_pagerank_binned_edge_cases(ctx, oracle):
     # a hub whose out-edges reach every bin, isolated vertices, V not a
     # multiple of the bin width, iteration counts 0 / 1 / 20, split bins
