@@ -333,6 +333,33 @@ def test_pagerank_binned_bits_and_tasks_do_not_change_ranks(ctx):
             assert (r.view(np.uint32) == ref.view(np.uint32)).all(), (bits, te)
 
 
+@pytest.mark.skip(reason="HIP-graph capture of the PageRank loop withdrawn: in its one GPU run the "
+                         "first BFS after a graph replay hung (no fault); cause not found, so "
+                         "the loop launches eagerly and this case stays off the GPU")
+def test_pagerank_binned_graph_replays(ctx):
+    # the iteration loop is captured into a HIP graph and replayed; a
+    # CU-masked context (mask = every CU) runs the same loop eagerly.  Every
+    # replay, with other work and re-captures in between, must equal it.
+    total = ctx.cu_count[1]
+    every_cu = [0xFFFFFFFF] * ((total + 31) // 32)
+    with grus.Graph.rmat(ctx, 14) as g, grus.Graph.rmat(ctx, 12) as h:
+        ctx.set_cu_mask(every_cu)
+        try:
+            eager20, _ = pagerank_with(ctx, g, 4)
+            eager7, _ = pagerank_with(ctx, g, 4, iters=7)
+            eager_h, _ = pagerank_with(ctx, h, 4)
+        finally:
+            ctx.set_cu_mask(None)
+        for order in ((20, 20, 20), (7, 7), (20,)):
+            for it in order:
+                r, st = pagerank_with(ctx, g, 4, iters=it)
+                ref = eager20 if it == 20 else eager7
+                assert (r.view(np.uint32) == ref.view(np.uint32)).all(), it
+                grus.bfs(g, g.info.max_degree_vertex)  # eager work between replays
+            r, _ = pagerank_with(ctx, h, 4)  # another graph's own capture
+            assert (r.view(np.uint32) == eager_h.view(np.uint32)).all()
+
+
 def test_pagerank_binned_edge_cases(ctx, oracle):
     # a hub whose out-edges reach every bin, isolated vertices, V not a
     # multiple of the bin width, iteration counts 0 / 1 / 20, split bins
