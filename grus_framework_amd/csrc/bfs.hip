@@ -38,6 +38,8 @@ namespace mgg {
 
 struct BfsWork {
   Buf visited, front[2], q[2], longq;
+  Buf first_in;                     // first in-neighbour of every vertex (GRUS_NO_VERTEX if none)
+  const void *first_for = nullptr;  // in-edge columns first_in was built from
   PushWork push;
   uint64_t cap_v = 0;
   void ensure(uint64_t V) {
@@ -96,15 +98,28 @@ __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uin
   }
 }
 
+// first[v] = lowest-id in-neighbour of v (rows are sorted), GRUS_NO_VERTEX
+// for an empty row
+__global__ void k_first_in(CsrView in, uint32_t *first) {
+  uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; v < in.V; v += stride)
+    first[v] = in.off[v] < in.off[v + 1] ? edge_col(in, in.off[v]) : GRUS_NO_VERTEX;
+}
+
 // bottom-up step over every bitmap word (see file comment).  A wave takes
-// PULL_UNROLL consecutive words (256 vertices) per step and issues their
-// visited words, offsets, first in-neighbours and frontier bits together,
-// so each lane keeps PULL_UNROLL independent load chains in flight; most
-// RMAT vertices find a parent at their first (lowest-id) in-neighbour.
+// PULL_UNROLL consecutive words (512 vertices) per step and issues their
+// visited words, first in-neighbours and row offsets together -- all three
+// coalesced and independent of each other -- then the frontier bits of the
+// first in-neighbours, so the first test of a vertex is two memory
+// latencies, the second an L2-resident bitmap word (reading the first
+// neighbour through the offsets added a dependent, uncoalesced column
+// load).  Most RMAT vertices find a parent at their first (lowest-id)
+// in-neighbour.
 constexpr int PULL_UNROLL = 8;
 constexpr int PULL_BATCH = 4;  // in-edges tested per dependent load round
 __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_off,
-                                                const uint32_t *first_col, const uint64_t *front,
+                                                const uint32_t *first_in, const uint64_t *front,
                                                 uint64_t *visited, uint64_t *front_next,
                                                 uint32_t *label, uint32_t next_level, uint32_t K,
                                                 uint32_t *longq, StepCounters *cnt, uint64_t W) {
@@ -117,30 +132,23 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
     uint64_t vis[PULL_UNROLL], beg[PULL_UNROLL], end[PULL_UNROLL];
     bool active[PULL_UNROLL], found[PULL_UNROLL];
     uint32_t first[PULL_UNROLL];
+    // visited words, first in-neighbours and offsets: independent loads
 #pragma unroll
-    for (int j = 0; j < PULL_UNROLL; j++)
+    for (int j = 0; j < PULL_UNROLL; j++) {
+      const uint64_t v = (w0 + j) * 64 + lane;
+      const bool in_range = v < in.V;
       vis[j] = (w0 + j < W) ? visited[w0 + j] : ~0ull;
+      first[j] = in_range ? first_in[v] : GRUS_NO_VERTEX;
+      beg[j] = in_range ? in.off[v] : 0;
+      end[j] = in_range ? in.off[v + 1] : 0;
+    }
 #pragma unroll
     for (int j = 0; j < PULL_UNROLL; j++) {
-      // pull_selector (bfs.cu:80-87): label == INFINIT, read as a clear visited bit
+      // pull_selector (bfs.cu:80-87): label == INFINIT, read as a clear
+      // visited bit (bits past V are set at init)
       active[j] = ((vis[j] >> lane) & 1ull) == 0;
-      uint32_t v = (uint32_t)((w0 + j) * 64 + lane);
-      beg[j] = end[j] = 0;
-      if (active[j]) {
-        beg[j] = in.off[v];
-        end[j] = in.off[v + 1];
-      }
+      found[j] = active[j] && first[j] != GRUS_NO_VERTEX && bm_test(front, first[j]);
     }
-    // the first neighbour: from the resident copy when the rows may live in
-    // host memory (capped placement), else from the column array
-#pragma unroll
-    for (int j = 0; j < PULL_UNROLL; j++) {
-      const uint32_t v = (uint32_t)((w0 + j) * 64 + lane);
-      first[j] = (beg[j] < end[j]) ? (first_col ? first_col[v] : edge_col(in, beg[j])) : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < PULL_UNROLL; j++)
-      found[j] = (beg[j] < end[j]) && bm_test(front, first[j]);
 #pragma unroll
     for (int j = 0; j < PULL_UNROLL; j++) {
       if (vis[j] == ~0ull) {
@@ -245,6 +253,23 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
   const CsrView in = (dir == GRUS_DIR_PUSH) ? out : g.csc();
   // push-only runs need no in-degrees (and so no CSC of a directed graph)
   const uint64_t *unreach = (dir == GRUS_DIR_PUSH) ? nullptr : g.unreachable();
+  // first in-neighbours for the bottom-up first test: the capped placement
+  // keeps its own resident copy; otherwise built once per in-edge array
+  const uint32_t *first_in = nullptr;
+  if (dir != GRUS_DIR_PUSH) {
+    if (g.first_col.p) {
+      first_in = g.first_col.as<uint32_t>();
+    } else {
+      if (w.first_for != in.col || w.first_in.bytes < g.V * 4) {
+        w.first_in.alloc(std::max<uint64_t>(g.V, 1) * 4);
+        bfs::k_first_in<<<grid_cap(ctx, g.V, BLOCK), BLOCK, 0, ctx.stream>>>(
+            in, w.first_in.as<uint32_t>());
+        GRUS_LAUNCH_CHECK();
+        w.first_for = in.col;
+      }
+      first_in = w.first_in.as<uint32_t>();
+    }
+  }
   uint32_t *label = g.result.as<uint32_t>();
   auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
   const double alpha = ctx.tuning.bfs_alpha > 0 ? ctx.tuning.bfs_alpha : 14.0;
@@ -289,7 +314,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
         launches += 2;
       }
       bfs::k_pull<<<grid_cap(ctx, div_up(W, bfs::PULL_UNROLL) * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          in, out.off, g.first_col.as<uint32_t>(), w.front[cur].as<uint64_t>(),
+          in, out.off, first_in, w.front[cur].as<uint64_t>(),
           w.visited.as<uint64_t>(), w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K,
           w.longq.as<uint32_t>(), cnt, W);
       GRUS_LAUNCH_CHECK();

@@ -23,6 +23,8 @@ using weight_t = uint32_t;
 using eid_t = uint64_t;
 
 constexpr uint32_t INFINIT = GRUS_INFINIT;
+// "no neighbour": the first-neighbour arrays hold it for empty rows
+#define GRUS_NO_VERTEX 0xFFFFFFFFu
 constexpr int WAVE = 64;           // CDNA wavefront width
 constexpr int BLOCK = 256;         // default workgroup: 4 waves
 constexpr int NXCD = 8;            // MI355X: 8 XCDs, each with its own L2

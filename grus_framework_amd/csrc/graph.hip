@@ -148,7 +148,7 @@ namespace {
 __global__ void k_first_col(const uint64_t *off, const uint32_t *col, uint64_t V, uint32_t *out) {
   uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; v < V; v += stride) out[v] = off[v] < off[v + 1] ? col[off[v]] : 0u;
+  for (; v < V; v += stride) out[v] = off[v] < off[v + 1] ? col[off[v]] : GRUS_NO_VERTEX;
 }
 
 // ---- derived quantities ----

@@ -112,8 +112,9 @@ struct Graph {
   int placement = GRUS_PLACE_HBM;
   Buf off, col, w;           // CSR (device, managed, or HBM prefix in capped mode)
   Buf col_far, w_far;        // capped mode: host-resident edge remainder
-  Buf first_col;             // capped mode: first neighbour of every row, in HBM,
-                             // so a bottom-up first test never crosses the host link
+  Buf first_col;             // capped mode: first neighbour of every row (GRUS_NO_VERTEX
+                             // if empty), in HBM, so a bottom-up first test never
+                             // crosses the host link
   uint64_t split = 0;        // first edge held in col_far
   Buf in_off, in_col, in_w;  // CSC of a directed graph (built on first pull use)
   bool csc_ready = false;
