@@ -39,13 +39,14 @@ __global__ void k_queue_to_bitmap(const uint32_t *q, uint32_t n, uint64_t *bm) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bitmap_to_queue(const uint64_t *bm, uint64_t W,
-                                                            uint32_t *q, uint32_t *count) {
+                                                            uint32_t *q, uint32_t *count,
+                                                            uint64_t flip) {
   __shared__ uint32_t s_wave[BLOCK / WAVE + 1];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   // block-uniform trip count; one lane per bitmap word
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < W; base += stride) {
     uint64_t wd = base + threadIdx.x;
-    uint64_t m = wd < W ? bm[wd] : 0ull;
+    uint64_t m = wd < W ? bm[wd] ^ flip : 0ull;
     uint32_t c = (uint32_t)__popcll(m);
     uint32_t x = c;  // inclusive wave scan of the popcounts
 #pragma unroll
@@ -139,7 +140,13 @@ void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uin
 }
 
 void bitmap_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count) {
-  k_bitmap_to_queue<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(bm, W, q, count);
+  k_bitmap_to_queue<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(bm, W, q, count, 0ull);
+  GRUS_LAUNCH_CHECK();
+}
+
+void clear_bits_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q,
+                         uint32_t *count) {
+  k_bitmap_to_queue<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(bm, W, q, count, ~0ull);
   GRUS_LAUNCH_CHECK();
 }
 

@@ -33,8 +33,11 @@ struct StepCounters {
   uint64_t next_edges;   // sum of out-degrees of the next frontier
   uint32_t far_count;    // vertices appended to the far queue (SSSP)
   uint32_t min_far;      // minimum live far distance (SSSP threshold advance)
-  uint64_t pad[5];
+  uint32_t rem_count;    // BFS bottom-up: vertices still unvisited after the step
+  uint32_t pad32;
+  uint64_t pad[4];
 };
+static_assert(sizeof(StepCounters) == 64, "counter block layout");
 
 __device__ __forceinline__ uint32_t edge_col(const CsrView &g, uint64_t e) {
   return e < g.split ? g.col[e] : g.col_far[e - g.split];
@@ -49,6 +52,9 @@ __device__ __forceinline__ uint32_t edge_w(const CsrView &g, uint64_t e) {
 // q <- every set bit of bm[0..W) in ascending order within each workgroup;
 // *count (device) receives the number appended (must be zero on entry)
 void bitmap_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count);
+// the same for every clear bit of bm[0..W) (bits past the last vertex must be set)
+void clear_bits_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q,
+                         uint32_t *count);
 // bm[0..W) <- bitmap of q[0..n)
 void queue_to_bitmap(Context &ctx, const uint32_t *q, uint32_t n, uint64_t *bm, uint64_t W);
 void fill_u32(Context &ctx, uint32_t *a, uint64_t n, uint32_t v);

@@ -38,6 +38,7 @@ namespace mgg {
 
 struct BfsWork {
   Buf visited, front[2], q[2], longq;
+  Buf cand;  // sparse bottom-up step: the unvisited vertices
   Buf first_in;                     // first in-neighbour of every vertex (GRUS_NO_VERTEX if none)
   const void *first_for = nullptr;  // in-edge columns first_in was built from
   PushWork push;
@@ -51,6 +52,7 @@ struct BfsWork {
     q[0].alloc((V + 1) * 4);
     q[1].alloc((V + 1) * 4);
     longq.alloc((V + 1) * 4);
+    cand.alloc((V + 1) * 4);
     push.ensure(V);
     cap_v = V;
   }
@@ -198,6 +200,85 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
   }
 }
 
+// Bottom-up step over a candidate list instead of the whole bitmap, for
+// late steps with few unvisited vertices left: the list is every clear bit
+// of the visited bitmap (in-degree-0 vertices and the bits past V are set
+// at init), compacted just before, its length *ncp on the device.  A lane
+// per candidate, the same first test and batched scan as k_pull; found
+// vertices set their visited and next-frontier bits with atomics
+// (candidates of one word are spread over lanes), front_next must be zero
+// on entry.  (Reading every vertex's first neighbour and offsets, as the
+// dense step does, costs ~20 bytes per vertex whatever is left.)
+constexpr int SPARSE_UNROLL = 4;
+constexpr uint64_t SPARSE_DIV = 16;  // scan the list when it holds under V / 16 vertices
+__global__ __launch_bounds__(BLOCK) void k_pull_sparse(
+    CsrView in, const uint64_t *out_off, const uint32_t *first_in, const uint64_t *front,
+    uint64_t *visited, uint64_t *front_next, uint32_t *label, uint32_t next_level, uint32_t K,
+    const uint32_t *cand, const uint32_t *ncp, uint32_t *longq, StepCounters *cnt) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  uint32_t lane = lane_id();
+  const uint32_t nc = *ncp;
+  uint32_t my_found = 0;
+  uint64_t my_edges = 0;
+  for (uint64_t c0 = wave * SPARSE_UNROLL * WAVE; c0 < nc; c0 += nwaves * SPARSE_UNROLL * WAVE) {
+    uint32_t v[SPARSE_UNROLL], first[SPARSE_UNROLL];
+    uint64_t beg[SPARSE_UNROLL], end[SPARSE_UNROLL];
+    bool valid[SPARSE_UNROLL], found[SPARSE_UNROLL];
+#pragma unroll
+    for (int j = 0; j < SPARSE_UNROLL; j++) {
+      const uint64_t i = c0 + (uint64_t)j * WAVE + lane;
+      valid[j] = i < nc;
+      v[j] = valid[j] ? cand[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < SPARSE_UNROLL; j++) {
+      first[j] = valid[j] ? first_in[v[j]] : GRUS_NO_VERTEX;
+      beg[j] = valid[j] ? in.off[v[j]] : 0;
+      end[j] = valid[j] ? in.off[v[j] + 1] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SPARSE_UNROLL; j++)
+      found[j] = first[j] != GRUS_NO_VERTEX && bm_test(front, first[j]);
+#pragma unroll
+    for (int j = 0; j < SPARSE_UNROLL; j++) {
+      bool is_long = false;
+      if (valid[j] && !found[j] && beg[j] < end[j]) {
+        uint64_t e = beg[j] + 1;
+        uint64_t lim = (end[j] - beg[j] > K) ? beg[j] + K : end[j];
+        for (; e < lim && !found[j]; e += PULL_BATCH) {
+          uint32_t c[PULL_BATCH];
+#pragma unroll
+          for (int q = 0; q < PULL_BATCH; q++) c[q] = (e + q < lim) ? edge_col(in, e + q) : 0u;
+          uint64_t fw[PULL_BATCH];
+#pragma unroll
+          for (int q = 0; q < PULL_BATCH; q++) fw[q] = (e + q < lim) ? front[c[q] >> 6] : 0ull;
+#pragma unroll
+          for (int q = 0; q < PULL_BATCH; q++) found[j] = found[j] || ((fw[q] >> (c[q] & 63)) & 1ull);
+        }
+        is_long = !found[j] && lim < end[j];
+      }
+      if (found[j]) {
+        const uint32_t u = v[j];
+        label[u] = next_level;
+        const unsigned long long bit = 1ull << (u & 63);
+        atomicOr((unsigned long long *)&front_next[u >> 6], bit);
+        atomicOr((unsigned long long *)&visited[u >> 6], bit);
+        my_found++;
+        my_edges += (out_off == in.off) ? end[j] - beg[j] : out_off[u + 1] - out_off[u];
+      }
+      const uint32_t slot = wave_append(&cnt->long_count, is_long);
+      if (is_long) longq[slot] = v[j];
+    }
+  }
+  my_found = wave_sum(my_found);
+  my_edges = wave_sum(my_edges);
+  if (lane == 0 && my_found) {
+    atomicAdd(&cnt->next_count, my_found);
+    atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+  }
+}
+
 // wave-cooperative scan of the vertices whose first K in-edges held no parent
 __global__ __launch_bounds__(BLOCK) void k_pull_long(CsrView in, const uint64_t *out_off,
                                                      const uint64_t *front, uint64_t *visited,
@@ -297,6 +378,10 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
   uint64_t mu = g.E >= src_deg ? g.E - src_deg : 0;
   uint64_t v_reached = 1, e_reached = src_deg;
   uint32_t level = 0, pull_levels = 0;
+  // a bottom-up step with few unvisited vertices left scans a list of them
+  // instead of every bitmap word; the host knows how many are left from
+  // the vertices reached so far and the in-degree-0 count
+  const uint64_t n_unreach = unreach ? g.unreachable_count : 0;
   // GRUS_TRACE=1 prints one line per level (mode, frontier, host time)
   static const bool trace = getenv("GRUS_TRACE") != nullptr;
   auto t_level = std::chrono::steady_clock::now();
@@ -308,21 +393,40 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
       else if (pull && (double)nf < (double)V / beta && nf < prev_nf) pull = false;
     }
     GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+    bool sparse = false;
     if (pull) {
       if (!have_bitmap) {
         queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nf, w.front[cur].as<uint64_t>(), W);
         launches += 2;
       }
-      bfs::k_pull<<<grid_cap(ctx, div_up(W, bfs::PULL_UNROLL) * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          in, out.off, first_in, w.front[cur].as<uint64_t>(),
-          w.visited.as<uint64_t>(), w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K,
-          w.longq.as<uint32_t>(), cnt, W);
-      GRUS_LAUNCH_CHECK();
+      // unvisited vertices left (an upper bound: the source may be one of
+      // the in-degree-0 vertices)
+      const uint64_t left = V + 1 > v_reached + n_unreach ? V + 1 - v_reached - n_unreach : 0;
+      sparse = left * bfs::SPARSE_DIV < V;
+      if (sparse) {
+        fill_u64(ctx, w.front[cur ^ 1].as<uint64_t>(), W, 0ull);
+        clear_bits_to_queue(ctx, w.visited.as<uint64_t>(), W, w.cand.as<uint32_t>(),
+                            &cnt->rem_count);
+        const uint64_t waves = div_up(std::max<uint64_t>(left, 1), (uint64_t)bfs::SPARSE_UNROLL * WAVE);
+        bfs::k_pull_sparse<<<grid_cap(ctx, waves * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+            in, out.off, first_in, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),
+            w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K, w.cand.as<uint32_t>(),
+            &cnt->rem_count, w.longq.as<uint32_t>(), cnt);
+        GRUS_LAUNCH_CHECK();
+        launches += 3;
+      } else {
+        bfs::k_pull<<<grid_cap(ctx, div_up(W, bfs::PULL_UNROLL) * WAVE, BLOCK, 8), BLOCK, 0,
+                      ctx.stream>>>(in, out.off, first_in, w.front[cur].as<uint64_t>(),
+                                    w.visited.as<uint64_t>(), w.front[cur ^ 1].as<uint64_t>(),
+                                    label, level + 1, K, w.longq.as<uint32_t>(), cnt, W);
+        GRUS_LAUNCH_CHECK();
+        launches++;
+      }
       bfs::k_pull_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
           in, out.off, w.front[cur].as<uint64_t>(), w.visited.as<uint64_t>(),
           w.front[cur ^ 1].as<uint64_t>(), label, level + 1, K, w.longq.as<uint32_t>(), cnt);
       GRUS_LAUNCH_CHECK();
-      launches += 2;
+      launches++;
       have_bitmap = true;
       have_queue = false;
       pull_levels++;
@@ -341,13 +445,13 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
       have_queue = true;
       have_bitmap = false;
     }
-    ctx.read_ctl(2);  // next_count | long_count, next_edges
+    ctx.read_ctl(4);  // next_count | long_count, next_edges, far | min_far, rem_count
     const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
     if (trace) {
       auto now = std::chrono::steady_clock::now();
-      fprintf(stderr, "bfs level %u %s nf=%llu mf=%llu -> next %u (long %u) %.1f us\n", level,
-              pull ? "pull" : "push", (unsigned long long)nf, (unsigned long long)mf,
-              hc->next_count, hc->long_count,
+      fprintf(stderr, "bfs level %u %s nf=%llu mf=%llu -> next %u (long %u, candidates %u) %.1f us\n",
+              level, pull ? (sparse ? "pull-sparse" : "pull") : "push", (unsigned long long)nf,
+              (unsigned long long)mf, hc->next_count, hc->long_count, sparse ? hc->rem_count : 0u,
               std::chrono::duration<double, std::micro>(now - t_level).count());
       t_level = now;
     }

@@ -188,6 +188,20 @@ __global__ void k_empty_rows_bitmap(const uint64_t *off, uint64_t V, uint64_t *b
   }
 }
 
+// set bits of bm among the first V
+__global__ void k_count_bits(const uint64_t *bm, uint64_t W, uint64_t V, unsigned long long *n) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long c = 0;
+  for (; i < W; i += stride) {
+    uint64_t m = bm[i];
+    if ((i + 1) * 64 > V) m &= (V % 64) ? ((1ull << (V % 64)) - 1ull) : ~0ull;
+    c += __popcll(m);
+  }
+  c = wave_sum(c);
+  if (lane_id() == 0 && c) atomicAdd(n, c);
+}
+
 // ---- RMAT ----
 __global__ void k_rmat_keys(RmatSpec s, uint64_t M, uint64_t *keys) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -380,6 +394,13 @@ const uint64_t *Graph::unreachable() {
     k_empty_rows_bitmap<<<grid_for(ctx, W * WAVE), BLOCK, 0, ctx->stream>>>(
         in.off, V, unreachable_bm.as<uint64_t>(), W);
     GRUS_LAUNCH_CHECK();
+    Buf n;
+    n.alloc(8);
+    GRUS_HIP(hipMemsetAsync(n.p, 0, 8, ctx->stream));
+    k_count_bits<<<grid_for(ctx, W), BLOCK, 0, ctx->stream>>>(unreachable_bm.as<uint64_t>(), W,
+                                                             V, n.as<unsigned long long>());
+    GRUS_LAUNCH_CHECK();
+    GRUS_HIP(hipMemcpyAsync(&unreachable_count, n.p, 8, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
     unreachable_ready = true;
   }

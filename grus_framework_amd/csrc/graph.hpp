@@ -124,6 +124,7 @@ struct Graph {
   uint64_t isolated = 0;
   Buf unreachable_bm;  // bit v set: in-degree 0 (never discovered by a traversal)
   bool unreachable_ready = false;
+  uint64_t unreachable_count = 0;  // vertices with in-degree 0 (set by unreachable())
   uint64_t host_bytes = 0;
   // workspaces and last result
   Buf result;
