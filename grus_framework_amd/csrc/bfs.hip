@@ -210,7 +210,9 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
 // on entry.  (Reading every vertex's first neighbour and offsets, as the
 // dense step does, costs ~20 bytes per vertex whatever is left.)
 constexpr int SPARSE_UNROLL = 4;
-constexpr uint64_t SPARSE_DIV = 16;  // scan the list when it holds under V / 16 vertices
+// scan the list when it holds under V / 16 vertices (RMAT s24, pinned
+// thresholds: V/4 507, V/8 531, V/16 534, V/32 518 GTEPS)
+constexpr uint64_t SPARSE_DIV = 16;
 __global__ __launch_bounds__(BLOCK) void k_pull_sparse(
     CsrView in, const uint64_t *out_off, const uint32_t *first_in, const uint64_t *front,
     uint64_t *visited, uint64_t *front_next, uint32_t *label, uint32_t next_level, uint32_t K,
