@@ -111,6 +111,66 @@ def test_bfs_directed_graph_all_directions(ctx, oracle):
                 assert (lv == ref).all(), (s, direction)
 
 
+def layered_graph_with_unreachable_block(rng):
+    """Directed: source 0 (in-degree 0) -> layers of 150000, 20000, 5000,
+    1000 vertices, each vertex with 2 parents in the layer above; an
+    unreachable strongly connected block U at the lowest ids (1..2000)
+    feeds 3 extra in-edges into half of the layer vertices (so their first
+    in-neighbour is often in U, never in a frontier), and 60 layer-3
+    vertices get 100 in-edges from U (rows past the bottom-up cut-off)."""
+    nu, sizes = 2000, [150000, 20000, 5000, 1000]
+    bounds = np.cumsum([1 + nu] + sizes)  # layer i holds ids [bounds[i], bounds[i + 1])
+    V = int(bounds[-1])
+    layer = [np.arange(bounds[i], bounds[i + 1]) for i in range(len(sizes))]
+    U = np.arange(1, 1 + nu)
+    us, vs = [np.zeros(sizes[0], np.int64)], [layer[0]]
+    for i in range(1, len(sizes)):
+        for _ in range(2):
+            us.append(rng.choice(layer[i - 1], sizes[i]))
+            vs.append(layer[i])
+    for L in layer:
+        half = rng.choice(L, len(L) // 2, replace=False)
+        for _ in range(3):
+            us.append(rng.choice(U, len(half)))
+            vs.append(half)
+    longv = rng.choice(layer[2], 60, replace=False)
+    us.append(rng.choice(U, 60 * 100))
+    vs.append(np.repeat(longv, 100))
+    us.append(U)
+    vs.append(np.roll(U, -1))  # a cycle through U
+    us.append(rng.choice(U, 8000))
+    vs.append(rng.choice(U, 8000))
+    u = np.concatenate(us).astype(np.uint64)
+    v = np.concatenate(vs).astype(np.uint64)
+    key = np.unique(u << np.uint64(32) | v)
+    key = key[(key >> np.uint64(32)) != (key & np.uint64(0xFFFFFFFF))]
+    src = (key >> np.uint64(32)).astype(np.int64)
+    col = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    off = np.zeros(V + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    return np.cumsum(off).astype(np.uint64), col
+
+
+def test_bfs_sparse_bottom_up_steps(ctx, oracle):
+    # late bottom-up steps scan a list of the unvisited vertices; here that
+    # list keeps the unreachable block, first tests miss into it and some
+    # rows go to the wave-cooperative scan
+    off, col = layered_graph_with_unreachable_block(np.random.default_rng(23))
+    h = oracle.CSR(off, col)
+    ref = oracle.bfs(h, 0)
+    assert (ref[1:2001] == INF).all() and ref[ref != INF].max() == 4
+    assert (ref != INF).sum() == 176001
+    with grus.Graph.from_csr(ctx, off, col, symmetric=False) as g:
+        for direction in ("pull", "auto"):
+            lv, st = grus.bfs(g, 0, direction)
+            assert (lv == ref).all(), direction
+            assert st.vertices_reached == (ref != INF).sum()
+        for s in (int(np.nonzero(ref == 1)[0][5]), 1):  # a layer-1 vertex; a vertex of U
+            r = oracle.bfs(h, s)
+            lv, _ = grus.bfs(g, s, "pull")
+            assert (lv == r).all(), s
+
+
 def test_bfs_closed_forms_and_edge_cases(ctx):
     # path: level = index
     n = 3000
