@@ -155,6 +155,94 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
   if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
 }
 
+// The same push for a small frontier (F <= PUSH_SMALL_F) in one launch:
+// every workgroup stages the whole queue, its row starts and degrees in LDS
+// and prefix-sums the degrees itself, so no degree kernel and device-wide
+// scan (three launches and their gaps) precede the expansion.  total (the
+// frontier's out-degree sum, known to the caller) sizes the grid; the
+// kernel walks the degree sum it computes itself.
+constexpr uint32_t PUSH_SMALL_F = 1024;
+template <class Job>
+__global__ __launch_bounds__(BLOCK) void k_push_small(CsrView g, const uint32_t *q, uint32_t F,
+                                                      uint64_t total, Job job, uint32_t *qn,
+                                                      uint32_t *qfar, StepCounters *cnt) {
+  __shared__ uint64_t sS[PUSH_SMALL_F + 1];  // exclusive prefix of the degrees
+  __shared__ uint64_t sRow[PUSH_SMALL_F];
+  __shared__ uint32_t sQ[PUSH_SMALL_F];
+  __shared__ uint64_t s_wsum[BLOCK / WAVE];
+  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
+  constexpr uint32_t PER = PUSH_SMALL_F / BLOCK;  // queue entries per thread
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  // stage: thread t owns entries [t*PER, t*PER + PER)
+  uint64_t d[PER];
+  uint64_t mine = 0;
+#pragma unroll
+  for (int i = 0; i < (int)PER; i++) {
+    const uint32_t t = threadIdx.x * PER + i;
+    d[i] = 0;
+    if (t < F) {
+      const uint32_t u = q[t];
+      const uint64_t a = g.off[u];
+      sQ[t] = u;
+      sRow[t] = a;
+      d[i] = g.off[u + 1] - a;
+    }
+    mine += d[i];
+  }
+  // exclusive block scan of the per-thread sums (wave shuffles, then LDS)
+  uint64_t x = mine;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if ((int)lane >= o) x += y;
+  }
+  if (lane == WAVE - 1) s_wsum[wid] = x;
+  __syncthreads();
+  uint64_t before = 0, sum = 0;
+  for (uint32_t w2 = 0; w2 < BLOCK / WAVE; w2++) {
+    before += (w2 < wid) ? s_wsum[w2] : 0ull;
+    sum += s_wsum[w2];
+  }
+  uint64_t run = before + x - mine;
+#pragma unroll
+  for (int i = 0; i < (int)PER; i++) {
+    const uint32_t t = threadIdx.x * PER + i;
+    if (t < F) sS[t] = run;
+    run += d[i];
+  }
+  if (threadIdx.x == 0) sS[F] = sum;
+  __syncthreads();
+  // the edge space is the staged degree sum (total only sized the grid)
+  uint64_t my_edges = 0;
+  // every thread of the block runs the same trip count (block_append)
+  for (uint64_t kb = (uint64_t)blockIdx.x * blockDim.x; kb < sum;
+       kb += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = kb + threadIdx.x;
+    int gen = 0;
+    uint32_t dst = 0;
+    if (k < sum) {
+      const uint32_t j = upper_bound_u64(sS, 0, F + 1, k) - 1;
+      const uint32_t u = sQ[j];
+      const uint64_t e = sRow[j] + (k - sS[j]);
+      dst = edge_col(g, e);
+      uint32_t val = 0;
+      const bool upd = job.update(u, dst, e, edge_w(g, e), &val);
+      gen = job.generate(upd, dst, val);
+    }
+    const uint32_t slot = block_append(&cnt->next_count, gen == 1, s_app);
+    if (gen == 1) {
+      qn[slot] = dst;
+      my_edges += job.out_degree(dst);
+    }
+    if (Job::kFar) {
+      const uint32_t fslot = block_append(&cnt->far_count, gen == 2, s_app);
+      if (gen == 2) qfar[fslot] = dst;
+    }
+  }
+  my_edges = wave_sum(my_edges);
+  if (lane == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+}
+
 // Host side of the push operator: degree scan + edge-balanced expansion
 // (PushWork, the scan buffers, is declared in graph.hpp).
 
@@ -170,6 +258,18 @@ void push_advance(Context &ctx, PushWork &w, const CsrView &g, const uint32_t *q
   if (F == 0 || total == 0) return;
   k_push_lb<Job><<<push_grid(ctx, total), BLOCK, 0, ctx.stream>>>(
       g, q, w.scan.as<uint64_t>(), F, total, job, qn, qfar, cnt);
+  GRUS_LAUNCH_CHECK();
+}
+
+// a frontier of at most PUSH_SMALL_F vertices: one launch (k_push_small)
+template <class Job>
+void push_advance_small(Context &ctx, const CsrView &g, const uint32_t *q, uint32_t F,
+                        uint64_t total, const Job &job, uint32_t *qn, StepCounters *cnt,
+                        uint32_t *qfar = nullptr) {
+  if (F == 0 || total == 0) return;
+  if (F > PUSH_SMALL_F) raise(GRUS_E_INVALID, "push_advance_small: frontier too large");
+  k_push_small<Job><<<grid_cap(ctx, total, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+      g, q, F, total, job, qn, qfar, cnt);
   GRUS_LAUNCH_CHECK();
 }
 

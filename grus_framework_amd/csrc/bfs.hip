@@ -439,11 +439,17 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
         GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
         launches += 1;
       }
-      push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nf);
       bfs::job_t job{label, w.visited.as<uint64_t>(), out.off, level + 1};
-      push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, job,
-                   w.q[cur ^ 1].as<uint32_t>(), cnt);
-      launches += 3;
+      if (nf <= PUSH_SMALL_F) {  // one launch, the degree scan done in LDS
+        push_advance_small(ctx, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, job,
+                           w.q[cur ^ 1].as<uint32_t>(), cnt);
+        launches += 1;
+      } else {
+        push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nf);
+        push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, job,
+                     w.q[cur ^ 1].as<uint32_t>(), cnt);
+        launches += 3;
+      }
       have_queue = true;
       have_bitmap = false;
     }
