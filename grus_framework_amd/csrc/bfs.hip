@@ -366,6 +366,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
     src_deg = o[1] - o[0];
   }
   uint64_t launches = 0;
+  GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
   GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   bfs::k_init<<<grid_cap(ctx, std::max(V, W), BLOCK), BLOCK, 0, ctx.stream>>>(
       label, V, w.visited.as<uint64_t>(), unreach, W, src, w.q[0].as<uint32_t>());
@@ -394,8 +395,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
       if (!pull && (double)mf > (double)mu / alpha) pull = true;
       else if (pull && (double)nf < (double)V / beta && nf < prev_nf) pull = false;
     }
-    GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
-    bool sparse = false;
+    bool sparse = false;  // (the counters are zero: cleared before the loop, then by publish_ctl)
     if (pull) {
       if (!have_bitmap) {
         queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nf, w.front[cur].as<uint64_t>(), W);
@@ -453,7 +453,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
       have_queue = true;
       have_bitmap = false;
     }
-    ctx.read_ctl(4);  // next_count | long_count, next_edges, far | min_far, rem_count
+    ctx.publish_ctl(sizeof(StepCounters) / 8);  // read the step's counters and clear them
     const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
     if (trace) {
       auto now = std::chrono::steady_clock::now();

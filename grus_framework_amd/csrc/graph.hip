@@ -114,6 +114,23 @@ void Context::set_cu_mask(const uint32_t *mask, uint32_t words) {
   cu_count = words ? n : cu_total;
 }
 
+namespace {
+__global__ void k_publish_ctl(uint64_t *d, uint64_t *h, uint32_t n) {
+  const uint32_t i = threadIdx.x;
+  if (i < n) {
+    h[i] = d[i];
+    d[i] = 0;
+  }
+}
+}  // namespace
+
+void Context::publish_ctl(size_t n) {
+  if (n > 64) raise(GRUS_E_INVALID, "publish_ctl: at most 64 words");
+  k_publish_ctl<<<1, 64, 0, stream>>>(d_ctl.as<uint64_t>(), h_ctl, (uint32_t)n);
+  GRUS_LAUNCH_CHECK();
+  GRUS_HIP(hipStreamSynchronize(stream));
+}
+
 void Context::read_ctl(size_t n) {
   GRUS_HIP(hipMemcpyAsync(h_ctl, d_ctl.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
   GRUS_HIP(hipStreamSynchronize(stream));

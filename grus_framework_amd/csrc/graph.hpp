@@ -79,6 +79,10 @@ struct Context {
   void activate() const { GRUS_HIP(hipSetDevice(device)); }
   // copy the first n u64 of the device control block to h_ctl and wait
   void read_ctl(size_t n);
+  // one kernel copies the first n u64 of the control block to h_ctl and
+  // zeroes them on the device; then wait.  Replaces a copy and the next
+  // step's memset (two launches) in per-level read-backs.
+  void publish_ctl(size_t n);
   // replace the stream by one restricted to the CUs set in mask (words == 0:
   // all CUs); waits for the old stream first
   void set_cu_mask(const uint32_t *mask, uint32_t words);
