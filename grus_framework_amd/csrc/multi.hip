@@ -13,6 +13,7 @@
 // (front bit = the lane's distance at u improved last round); a wave owns a
 // chunk of up to 64 edges of one frontier vertex.
 #include <algorithm>
+#include <cstring>
 
 #include "advance.hpp"
 #include "device.hpp"
@@ -302,6 +303,15 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, ui
   if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
 }
 
+// the step's counters to the pinned host mirror, cleared for the next step
+__global__ void k_publish(uint64_t *d, uint64_t *h, uint32_t n) {
+  const uint32_t i = threadIdx.x;
+  if (i < n) {
+    h[i] = d[i];
+    d[i] = 0;
+  }
+}
+
 // ---- extraction ----
 __global__ void k_gather_col(const uint32_t *dist, uint64_t V, uint32_t lane, uint32_t *row) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -419,6 +429,7 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
   srcs_dev_.alloc(1024);
   cnt_.alloc(sizeof(MsCounters));
   push_cnt_.alloc(sizeof(StepCounters));
+  hmirror_.alloc(256, Mem::HostPinned);
   if (kind_ == MsKind::BFS) {
     vals_.alloc((uint64_t)lanes_ * V * 4 + 8);  // level[lane][v]
     fill_u64(ctx, visit_.as<uint64_t>(), V + 1, 0);
@@ -500,12 +511,33 @@ std::vector<int> MsEngine::admit(const uint32_t *srcs, int n) {
                                          front_.as<uint64_t>(), vals_.as<uint32_t>(), g_.V,
                                          queue_.as<uint32_t>(), cnt);
   GRUS_LAUNCH_CHECK();
-  MsCounters h;
-  GRUS_HIP(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, ctx.stream));
-  ctx.sync();
+  const MsCounters h = publish();
   qn_ = h.qcount;
   busy_ |= mask;
   return used;
+}
+
+// read the counters and clear them on the device in one launch (the next
+// step starts from zero counters without a memset)
+MsCounters MsEngine::publish() {
+  Context &ctx = *ctx_;
+  static_assert(sizeof(MsCounters) % 8 == 0 && sizeof(MsCounters) <= 128, "counter block");
+  ms::k_publish<<<1, 64, 0, ctx.stream>>>(cnt_.as<uint64_t>(), hmirror_.as<uint64_t>(),
+                                          (uint32_t)(sizeof(MsCounters) / 8));
+  GRUS_LAUNCH_CHECK();
+  ctx.sync();
+  MsCounters h;
+  memcpy(&h, hmirror_.p, sizeof h);
+  return h;
+}
+
+// one device u64 to the host through the pinned mirror (no pageable staging)
+uint64_t MsEngine::read_u64(const uint64_t *d) {
+  Context &ctx = *ctx_;
+  uint64_t *hm = hmirror_.as<uint64_t>() + 16;
+  GRUS_HIP(hipMemcpyAsync(hm, d, 8, hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  return *hm;
 }
 
 std::vector<int> MsEngine::step() {
@@ -520,7 +552,7 @@ std::vector<int> MsEngine::step() {
   // have a non-empty frontier: finished lanes are reported and leave the
   // running set at once)
   const uint64_t running = busy_ & ~finished_mask_;
-  GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(MsCounters), ctx.stream));
+  // the counters are zero here: cleared by the previous publish()
   if (kind_ == MsKind::BFS) {
     const CsrView in = g_.csc();
     const bool pull = (double)front_edges_ > (double)g_.E / 24.0;
@@ -535,17 +567,15 @@ std::vector<int> MsEngine::step() {
       GRUS_LAUNCH_CHECK();
     } else if (qn_) {
       push_scan_degrees(ctx, push_, out.off, queue_.as<uint32_t>(), (uint32_t)qn_);
-      uint64_t total = 0;
-      GRUS_HIP(hipMemcpyAsync(&total, push_.scan.as<uint64_t>() + qn_, 8, hipMemcpyDeviceToHost,
-                              ctx.stream));
-      ctx.sync();
+      const uint64_t total = read_u64(push_.scan.as<uint64_t>() + qn_);
       ms::push_job_t job{front_.as<uint64_t>(), visit_.as<uint64_t>(), next_.as<uint64_t>()};
       push_advance(ctx, push_, out, queue_.as<uint32_t>(), (uint32_t)qn_, total, job,
                    nullptr, push_cnt_.as<StepCounters>());
     }
     ms::LaneStart st;
     for (int i = 0; i < 64; i++) st.s[i] = start_[i];
-    GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(MsCounters), ctx.stream));
+    // (a pull step left long_count set; finalize uses the other counters,
+    // which are still zero)
     ms::k_bfs_finalize<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
         out.off, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(),
         vals_.as<uint32_t>(), V, running, (uint32_t)t_, st, queue_.as<uint32_t>(), cnt);
@@ -557,10 +587,7 @@ std::vector<int> MsEngine::step() {
           out.off, queue_.as<uint32_t>(), (uint32_t)qn_, push_.deg.as<uint64_t>());
       GRUS_LAUNCH_CHECK();
       scan_excl_u64(ctx, push_.deg.as<uint64_t>(), cs.as<uint64_t>(), qn_ + 1);
-      uint64_t chunks = 0;
-      GRUS_HIP(hipMemcpyAsync(&chunks, cs.as<uint64_t>() + qn_, 8, hipMemcpyDeviceToHost,
-                              ctx.stream));
-      ctx.sync();
+      const uint64_t chunks = read_u64(cs.as<uint64_t>() + qn_);
       if (chunks) {
         ms::k_sssp_relax<<<grid_cap(ctx, chunks * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
             out, queue_.as<uint32_t>(), (uint32_t)qn_, cs.as<uint64_t>(), chunks,
@@ -573,9 +600,7 @@ std::vector<int> MsEngine::step() {
         cnt);
     GRUS_LAUNCH_CHECK();
   }
-  MsCounters h;
-  GRUS_HIP(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, ctx.stream));
-  ctx.sync();
+  const MsCounters h = publish();
   t_++;
   qn_ = h.qcount;
   front_edges_ = h.front_edges;

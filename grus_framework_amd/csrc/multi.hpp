@@ -70,8 +70,11 @@ class MsEngine {
   uint64_t t_ = 0;         // global step counter
   uint64_t qn_ = 0, front_edges_ = 0;
   Buf visit_, front_, next_, vals_, queue_, longq_, row_, digest_, srcs_dev_, cnt_, push_cnt_;
+  Buf hmirror_;  // pinned host words: the counters as published, then read-back scratch
   PushWork push_;
   void reset_lanes(uint64_t mask);
+  MsCounters publish();
+  uint64_t read_u64(const uint64_t *d);
 };
 
 }  // namespace mgg
