@@ -38,6 +38,8 @@ namespace mgg {
 
 struct BfsWork {
   Buf visited, front[2], q[2], longq;
+  Buf stamp;            // large top-down steps: byte per vertex, = the step's stamp when found
+  uint8_t last_stamp = 0;
   Buf cand;  // sparse bottom-up step: the unvisited vertices
   Buf first_in;                     // first in-neighbour of every vertex (GRUS_NO_VERTEX if none)
   const void *first_for = nullptr;  // in-edge columns first_in was built from
@@ -47,6 +49,9 @@ struct BfsWork {
     if (V <= cap_v) return;
     uint64_t W = (V + 63) / 64;
     visited.alloc((W + 1) * 8);
+    stamp.alloc(W * 64 + 64);
+    GRUS_HIP(hipMemset(stamp.p, 0, W * 64 + 64));
+    last_stamp = 0;
     front[0].alloc((W + 1) * 8);
     front[1].alloc((W + 1) * 8);
     q[0].alloc((V + 1) * 4);
@@ -83,6 +88,72 @@ struct job_t {
     return out_off[v + 1] - out_off[v];
   }
 };
+
+// Updater of a large top-down step (many edges): no atomics.  An unvisited
+// destination gets its level and the step's stamp with plain stores; every
+// writer stores the same values, so concurrent writes agree.  Nothing is
+// appended: k_stamp_commit turns the stamps into the visited / frontier
+// bits and counts (one pass over V bytes instead of an atomic per claim).
+struct stamp_job_t {
+  static constexpr bool kFar = false;
+  uint32_t *label;
+  const uint64_t *visited;
+  uint8_t *stamp;
+  uint32_t next_level;
+  uint8_t step_stamp;
+  __device__ __forceinline__ bool update(vtx_t, vtx_t dst, eid_t, weight_t, uint32_t *) const {
+    if (bm_test(visited, dst)) return false;
+    label[dst] = next_level;
+    stamp[dst] = step_stamp;
+    return false;
+  }
+  __device__ __forceinline__ int generate(bool, vtx_t, uint32_t) const { return 0; }
+  __device__ __forceinline__ uint64_t out_degree(vtx_t) const { return 0; }
+};
+
+// after a stamped step: word w's new vertices = its bytes equal to the
+// stamp; they join visited and the next frontier bitmap (every word
+// written), and the counters get their number and out-degree sum
+__global__ __launch_bounds__(BLOCK) void k_stamp_commit(const uint8_t *stamp, uint8_t step_stamp,
+                                                        const uint64_t *out_off, uint64_t *visited,
+                                                        uint64_t *front_next, uint64_t W,
+                                                        uint64_t V, StepCounters *cnt) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t found = 0;
+  uint64_t edges = 0;
+  for (; i < W; i += stride) {
+    const uint64_t *b = reinterpret_cast<const uint64_t *>(stamp + i * 64);
+    const uint64_t rep = 0x0101010101010101ull * step_stamp;
+    uint64_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      // 8 bytes at a time: y has a zero byte where the stamp matches; the
+      // high bit of each byte of nz is set exactly where y's byte is not
+      // zero (no carry crosses a byte); the multiply gathers those 8 bits
+      const uint64_t y = b[k] ^ rep;
+      const uint64_t nz = (((y & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | y) &
+                          0x8080808080808080ull;
+      const uint64_t z = (nz ^ 0x8080808080808080ull) >> 7;  // bit 8j set: byte j matches
+      m |= ((z * 0x0102040810204080ull) >> 56) << (8 * k);
+    }
+    front_next[i] = m;
+    if (m) {
+      visited[i] |= m;
+      found += (uint32_t)__popcll(m);
+      for (uint64_t r = m; r; r &= r - 1) {
+        const uint64_t v = i * 64 + (uint64_t)(__ffsll((unsigned long long)r) - 1);
+        if (v < V) edges += out_off[v + 1] - out_off[v];
+      }
+    }
+  }
+  found = wave_sum(found);
+  edges = wave_sum(edges);
+  if (lane_id() == 0 && found) {
+    atomicAdd(&cnt->next_count, found);
+    atomicAdd((unsigned long long *)&cnt->next_edges, edges);
+  }
+}
 
 __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uint64_t *unreachable,
                        uint64_t W, uint32_t src, uint32_t *q0) {
@@ -213,6 +284,10 @@ constexpr int SPARSE_UNROLL = 4;
 // scan the list when it holds under V / 16 vertices (RMAT s24, pinned
 // thresholds: V/4 507, V/8 531, V/16 534, V/32 518 GTEPS)
 constexpr uint64_t SPARSE_DIV = 16;
+// a top-down step over more than V / STAMP_DIV edges claims with stamps
+// (RMAT s24, pinned thresholds: V/16 580, V/32 584, V/64 583, V/128 582
+// GTEPS; 551 without stamps)
+constexpr uint64_t STAMP_DIV = 64;
 __global__ __launch_bounds__(BLOCK) void k_pull_sparse(
     CsrView in, const uint64_t *out_off, const uint32_t *first_in, const uint64_t *front,
     uint64_t *visited, uint64_t *front_next, uint32_t *label, uint32_t next_level, uint32_t K,
@@ -396,6 +471,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
       else if (pull && (double)nf < (double)V / beta && nf < prev_nf) pull = false;
     }
     bool sparse = false;  // (the counters are zero: cleared before the loop, then by publish_ctl)
+    bool stamped = false;
     if (pull) {
       if (!have_bitmap) {
         queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nf, w.front[cur].as<uint64_t>(), W);
@@ -440,7 +516,32 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
         launches += 1;
       }
       bfs::job_t job{label, w.visited.as<uint64_t>(), out.off, level + 1};
-      if (nf <= PUSH_SMALL_F) {  // one launch, the degree scan done in LDS
+      if (mf * bfs::STAMP_DIV > V) {
+        // many edges: plain stores and one commit pass instead of an atomic
+        // claim per discovered vertex; leaves the frontier as a bitmap
+        if (++w.last_stamp == 0) {  // stamps wrapped: clear and restart at 1
+          GRUS_HIP(hipMemsetAsync(w.stamp.p, 0, W * 64, ctx.stream));
+          w.last_stamp = 1;
+        }
+        bfs::stamp_job_t sj{label, w.visited.as<uint64_t>(), w.stamp.as<uint8_t>(), level + 1,
+                            w.last_stamp};
+        if (nf <= PUSH_SMALL_F) {
+          push_advance_small(ctx, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, sj, nullptr,
+                             cnt);
+          launches += 1;
+        } else {
+          push_scan_degrees(ctx, w.push, out.off, w.q[cur].as<uint32_t>(), (uint32_t)nf);
+          push_advance(ctx, w.push, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, sj, nullptr,
+                       cnt);
+          launches += 3;
+        }
+        bfs::k_stamp_commit<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(
+            w.stamp.as<uint8_t>(), w.last_stamp, out.off, w.visited.as<uint64_t>(),
+            w.front[cur ^ 1].as<uint64_t>(), W, V, cnt);
+        GRUS_LAUNCH_CHECK();
+        launches++;
+        stamped = true;
+      } else if (nf <= PUSH_SMALL_F) {  // one launch, the degree scan done in LDS
         push_advance_small(ctx, out, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, job,
                            w.q[cur ^ 1].as<uint32_t>(), cnt);
         launches += 1;
@@ -450,8 +551,8 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
                      w.q[cur ^ 1].as<uint32_t>(), cnt);
         launches += 3;
       }
-      have_queue = true;
-      have_bitmap = false;
+      have_queue = !stamped;  // a stamped step leaves the next frontier as a bitmap only
+      have_bitmap = stamped;
     }
     ctx.publish_ctl(sizeof(StepCounters) / 8);  // read the step's counters and clear them
     const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
