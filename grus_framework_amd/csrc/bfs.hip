@@ -566,6 +566,9 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
     }
     prev_nf = nf;
     nf = hc->next_count;
+    // the next step reads nf queue entries: a count past V would be a
+    // kernel fault in the making, so it stops here
+    if (nf > V) raise(GRUS_E_HIP, "bfs step reported more new vertices than the graph has");
     mf = hc->next_edges;
     mu = mu >= mf ? mu - mf : 0;
     v_reached += nf;
