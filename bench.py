@@ -381,15 +381,20 @@ def run_cu_scaling(grus, ctx, g, args):
 
 
 def run_serving(grus):
-    """Batched serving on RMAT s22 (w in [1,255]) vs a one-query-at-a-time loop."""
+    """Batched serving on RMAT s22 (w in [1,255]) vs a one-query-at-a-time loop.
+
+    Queries: 50/50 BFS/SSSP, sources uniform over non-isolated vertices
+    (seed 5).  Saturation: all queries offered at once.  Latency: open-loop
+    Poisson arrivals (seed 4) at 30%, 60% and 90% of the measured saturation
+    rate, submit -> result-ready on the server's clock, 3400 queries each."""
     ctx = grus.Context(0)
     g = grus.Graph.rmat(ctx, 22, weights=(1, 255), wseed=3)
     off, _, _ = g.csr()
     nz = np.nonzero(np.diff(off.astype(np.int64)))[0]
     rng = np.random.default_rng(5)
-    N = 2048
-    kinds = rng.integers(0, 2, N)
-    srcs = rng.choice(nz, N)
+    N, NP = 2048, 3400
+    kinds = rng.integers(0, 2, NP)
+    srcs = rng.choice(nz, NP)
     # serial: best single-query kernels, one at a time, digest per query
     ns = 128
     lat = []
@@ -403,35 +408,53 @@ def run_serving(grus):
     res = {"serial_qps": round(serial_qps, 1),
            "serial_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
            "serial_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2)}
-    # batched at saturation: all queries offered at once
-    with grus.Server(g, max_batch=64, max_inflight_batches=8) as srv:
-        t0 = time.perf_counter()
-        for i in range(N):
-            srv.submit(int(kinds[i]), int(srcs[i]))
-        out = [srv.poll(300000) for _ in range(N)]
-        tb = time.perf_counter() - t0
-    assert all(r.status == 0 for r in out)
-    res["batched_qps"] = round(N / tb, 1)
-    res["speedup_vs_serial"] = round(N / tb / serial_qps, 2)
-    # open-loop Poisson arrivals at 60% of the measured batched capacity
-    rate = 0.6 * N / tb
-    n2 = 1024
-    gaps = rng.exponential(1.0 / rate, n2)
-    with grus.Server(g, max_batch=64, max_inflight_batches=8) as srv:
-        t0 = time.perf_counter()
-        due = t0
-        for i in range(n2):
-            due += gaps[i]
-            while time.perf_counter() < due:
-                pass
-            srv.submit(int(kinds[i]), int(srcs[i]))
-        out = [srv.poll(300000) for _ in range(n2)]
-    lat = np.array([(r.done_us - r.submit_us) / 1e3 for r in out])
-    res.update({"poisson_rate_qps": round(rate, 1), "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                "p99_ms": round(float(np.percentile(lat, 99)), 2), "queries": N + n2,
+
+    def saturate(**opts):
+        with grus.Server(g, **opts) as srv:
+            t0 = time.perf_counter()
+            for i in range(N):
+                srv.submit(int(kinds[i]), int(srcs[i]))
+            out = [srv.poll(300000) for _ in range(N)]
+            tb = time.perf_counter() - t0
+        assert all(r.status == 0 for r in out)
+        return N / tb
+
+    # the library's default policy: one engine per kind, a batch admitted
+    # into an idle engine; and lane-by-lane admission into 8 engines per kind
+    qps = saturate()
+    qps_cont = saturate(max_inflight_batches=8, admit_min_free=1)
+    res["batched_qps"] = round(qps, 1)
+    res["speedup_vs_serial"] = round(qps / serial_qps, 2)
+    res["policy"] = "1 engine per kind x 64 lanes, admit into idle engines"
+    res["continuous_8_engines_qps"] = round(qps_cont, 1)
+    sweep = []
+    total = N * 2
+    for frac in (0.3, 0.6, 0.9):
+        rate = frac * qps
+        gaps = np.random.default_rng(4).exponential(1.0 / rate, NP)
+        with grus.Server(g) as srv:
+            t0 = time.perf_counter()
+            due = t0
+            for i in range(NP):
+                due += gaps[i]
+                while time.perf_counter() < due:
+                    pass
+                srv.submit(int(kinds[i]), int(srcs[i]))
+            out = [srv.poll(300000) for _ in range(NP)]
+        assert all(r.status == 0 for r in out)
+        lat = np.array([(r.done_us - r.submit_us) / 1e3 for r in out])
+        sweep.append({"load": frac, "rate_qps": round(rate, 1),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2)})
+        total += NP
+    mid = sweep[1]
+    res.update({"poisson_rate_qps": mid["rate_qps"], "p50_ms": mid["p50_ms"],
+                "p99_ms": mid["p99_ms"], "latency_sweep": sweep, "queries": total,
                 "mix": "50/50 BFS/SSSP, sources uniform over non-isolated vertices"})
-    log("serving: serial %.1f qps, batched %.1f qps (x%.2f), p50 %.1f ms p99 %.1f ms at %.0f qps" %
-        (serial_qps, N / tb, N / tb / serial_qps, res["p50_ms"], res["p99_ms"], rate))
+    log("serving: serial %.1f qps, batched %.1f qps (x%.2f; lane-by-lane admission %.1f qps), "
+        "p50/p99 ms at 30/60/90%% load: %s" %
+        (serial_qps, qps, qps / serial_qps, qps_cont,
+         " ".join("%.0f/%.0f" % (x["p50_ms"], x["p99_ms"]) for x in sweep)))
     g.close()
     ctx.close()
     return res

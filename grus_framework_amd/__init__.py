@@ -292,11 +292,13 @@ def autotune(g, kind="bfs", srcs=(), cache=None):
 class Server:
     """Continuous-batching query server (grus_server_*)."""
 
-    def __init__(self, g, max_inflight_batches=4, max_batch=64, state_budget_bytes=0,
-                 copy_results=False):
+    def __init__(self, g, max_inflight_batches=0, max_batch=0, state_budget_bytes=0,
+                 copy_results=False, admit_min_free=0):
+        """Zero for max_inflight_batches, max_batch or admit_min_free picks the
+        library default (1 engine per kind, 64 lanes, admit into idle engines)."""
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
-                               int(copy_results))
+                               int(copy_results), admit_min_free)
         h = C.c_void_p()
         capi.check(_lib().grus_server_create(g.h, C.byref(opts), C.byref(h)))
         self.h = h

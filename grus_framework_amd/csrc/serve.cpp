@@ -5,10 +5,20 @@
 // engines, each owning 1..64 lane slots of device state and its own HIP
 // stream, driven by its own worker thread.  Between two steps of its engine
 // a worker admits waiting queries into free lanes (slot + state memory
-// available), so queries that arrive while others run join the batch in
-// flight, and engines of different kinds overlap on the device.  A lane whose
-// frontier empties is finished: its result is digested (and copied to the
-// caller's buffer when requested) and the slot is freed for the next query.
+// available) once at least admit_min_free lanes are free, so queries that
+// arrive while others run join the batch in flight, and engines of different
+// kinds overlap on the device.  A lane whose frontier empties is finished: its
+// result is digested (and copied to the caller's buffer when requested) and
+// the slot is freed for the next query.
+//
+// The default admits only into an idle engine.  Lanes of one batch then stay
+// in step: on small-world graphs their frontiers cover mostly the same
+// vertices at the same step, and a step of the engine costs about one sweep of
+// their union.  Lanes admitted one at a time drift apart, every step sweeps a
+// large union, and the engine spends its bandwidth on few queries at a time
+// (RMAT scale 22, 50/50 BFS/SSSP, saturated: 1290 queries/s for idle-only
+// admission against 1140 for any-free-lane admission with 8 engines per kind,
+// and p99 latency 0.2 s against 2.6 s at 60% load).
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -40,17 +50,21 @@ struct Query {
 class ServerImpl : public Server {
  public:
   ServerImpl(Graph &g, const grus_server_opts *o) : g_(g) {
-    opts_.max_inflight_batches = 4;
+    opts_.max_inflight_batches = 1;
     opts_.max_batch = 64;
     opts_.state_budget_bytes = 0;
     opts_.copy_results = 0;
+    opts_.admit_min_free = 0;
     if (o) {
       if (o->max_inflight_batches > 0) opts_.max_inflight_batches = o->max_inflight_batches;
       if (o->max_batch > 0) opts_.max_batch = o->max_batch;
       opts_.state_budget_bytes = o->state_budget_bytes;
       opts_.copy_results = o->copy_results;
+      opts_.admit_min_free = o->admit_min_free;
     }
     if (opts_.max_batch > 64) raise(GRUS_E_INVALID, "max_batch is at most 64");
+    if (opts_.admit_min_free < 0 || opts_.admit_min_free > opts_.max_batch)
+      raise(GRUS_E_INVALID, "admit_min_free is between 0 and max_batch");
     if (opts_.max_inflight_batches > 8) raise(GRUS_E_INVALID, "at most 8 engines per kind");
     // lanes per engine from the state budget: a BFS lane holds a V-entry
     // level row, an SSSP lane a V-entry distance column
@@ -64,6 +78,9 @@ class ServerImpl : public Server {
         raise(GRUS_E_BUDGET, "state budget too small for one lane per engine");
       lanes = (int)std::min<uint64_t>((uint64_t)lanes, (per_engine - shared) / per_lane);
     }
+    // admission gate, in lanes of an engine (capped by a budget-shrunk width)
+    min_free_ = opts_.admit_min_free ? opts_.admit_min_free : opts_.max_batch;
+    min_free_ = std::min(min_free_, lanes);
     if (!g.symmetric) g.csc();  // build the CSC once, before workers share it
     for (int k = 0; k < 2; k++)
       for (int e = 0; e < opts_.max_inflight_batches; e++)
@@ -134,6 +151,10 @@ class ServerImpl : public Server {
           cv_work_.wait(lk, [&] { return stop_ || !pending_[kind].empty() || eng.active(); });
           if (stop_ && !eng.active()) break;
           int room = stop_ ? 0 : eng.free_lanes();
+          // a running batch takes newcomers only once enough lanes are free:
+          // lanes admitted together keep their frontiers in step, so a step
+          // of the engine pays for one union of mostly shared vertices
+          if (eng.active() && room < min_free_) room = 0;
           while (room-- > 0 && !pending_[kind].empty()) {
             take.push_back(pending_[kind].front());
             pending_[kind].pop_front();
@@ -198,6 +219,7 @@ class ServerImpl : public Server {
   std::condition_variable cv_work_, cv_done_;
   std::deque<Query> pending_[2];
   std::deque<grus_query_result> done_;
+  int min_free_ = 1;
   bool stop_ = false, failed_ = false;
   int failed_code_ = GRUS_OK;
   std::string failed_msg_;

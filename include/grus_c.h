@@ -251,12 +251,22 @@ int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
 typedef struct grus_server_opts {
   int max_inflight_batches;   /* concurrent batches per query kind (engines,
                                  each with its own stream and worker thread),
-                                 1..8, default 4: more engines overlap more
-                                 of the per-step host round trips */
+                                 1..8, default 1: a traversal step already
+                                 covers the whole graph for a full batch, so
+                                 more engines mostly share the device */
   int max_batch;              /* lanes per batch, 1..64, default 64 */
   uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
   int copy_results;           /* 1: copy each result to the host buffer given
                                  at submit; 0: keep a digest only */
+  int admit_min_free;         /* an engine that still has running queries
+                                 admits waiting ones only once at least this
+                                 many of its lanes are free (an idle engine
+                                 always admits).  1 = admit into any free lane
+                                 at once (fully continuous); larger values keep
+                                 a batch's queries in step, which shrinks the
+                                 union of their frontiers.  0 = default:
+                                 max_batch (admit into an idle engine only);
+                                 at most max_batch */
 } grus_server_opts;
 typedef struct grus_query_result {
   uint64_t id;

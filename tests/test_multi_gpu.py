@@ -102,7 +102,12 @@ def single_digests(g, kind, srcs):
     return d
 
 
-def test_server_mix_matches_single_queries(ctx, oracle):
+@pytest.mark.parametrize("opts", [
+    {},  # default: one engine per kind, batches admitted into idle engines
+    {"max_inflight_batches": 3, "admit_min_free": 1},  # lane-by-lane admission
+    {"max_inflight_batches": 2, "max_batch": 16, "admit_min_free": 5},
+], ids=["idle-only", "any-free-lane", "gate5of16"])
+def test_server_mix_matches_single_queries(ctx, oracle, opts):
     with grus.Graph.rmat(ctx, 13, weights=(1, 255), wseed=3) as g:
         h = host_csr(g)
         rng = np.random.default_rng(11)
@@ -111,7 +116,7 @@ def test_server_mix_matches_single_queries(ctx, oracle):
         ref = {grus.Q_BFS: single_digests(g, grus.Q_BFS, [s for k, s in qs if k == 0]),
                grus.Q_SSSP: single_digests(g, grus.Q_SSSP, [s for k, s in qs if k == 1])}
         outs = {}
-        with grus.Server(g, copy_results=True) as srv:
+        with grus.Server(g, copy_results=True, **opts) as srv:
             ids = {}
             for kind, s in qs:
                 buf = np.zeros(g.V, np.uint32)
@@ -155,6 +160,10 @@ def test_server_errors(ctx):
         with pytest.raises(grus.GrusError) as e:
             grus.Server(g, state_budget_bytes=1024)
         assert e.value.code == capi.E_BUDGET
+        for bad in ({"admit_min_free": -1}, {"max_batch": 8, "admit_min_free": 9}):
+            with pytest.raises(grus.GrusError) as e:
+                grus.Server(g, **bad)
+            assert e.value.code == capi.E_INVALID
         with grus.Server(g, max_batch=4) as srv:
             with pytest.raises(grus.GrusError) as e:
                 srv.submit(grus.Q_BFS, g.V + 5)
