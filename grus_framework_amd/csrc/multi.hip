@@ -219,53 +219,76 @@ __global__ void k_chunk_counts(const uint64_t *off, const uint32_t *q, uint32_t 
   }
 }
 
+// Relaxation of one 64-edge chunk of frontier vertex u per wave task.  Lane l
+// loads edge e+l of the chunk (coalesced), and the edges are then broadcast to
+// the wave with readlane.  Distances only decrease within a round, so a plain
+// read of dist[v] that already shows nd below it proves that dist[v] drops
+// this round: the atomics are issued without waiting for their old values, and
+// several edges' row reads are in flight before the first atomic.
 __global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t *q, uint32_t F,
                                                       const uint64_t *CS, uint64_t chunks,
                                                       const uint64_t *front, uint64_t *next,
                                                       uint32_t *dist) {
-  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
   const uint32_t lane = lane_id();
-  for (uint64_t c = wave; c < chunks; c += nwaves) {
-    uint32_t j = upper_bound_u64(CS, 0, F + 1, c) - 1;
-    uint32_t u = q[j];
-    uint64_t e = g.off[u] + (c - CS[j]) * CHUNK;
+  // each wave takes a contiguous run of chunks (equal counts, so equal work
+  // within one chunk), finds the owner of its first chunk by binary search
+  // and then walks the chunk scan forward
+  const uint64_t per = (chunks + nwaves - 1) / nwaves;
+  const uint64_t c0 = wave * per;
+  const uint64_t c1 = c0 + per < chunks ? c0 + per : chunks;
+  if (c0 >= c1) return;
+  uint32_t j = upper_bound_u64(CS, 0, F + 1, c0) - 1;
+  for (uint64_t c = c0; c < c1; c++) {
+    while (CS[j + 1] <= c) j++;  // skips queued vertices without out-edges
+    const uint32_t u = q[j];
+    const uint64_t e = g.off[u] + (c - CS[j]) * CHUNK;
     uint64_t e1 = g.off[u + 1];
     if (e1 > e + CHUNK) e1 = e + CHUNK;
+    const uint32_t n = (uint32_t)(e1 - e);  // 1..64, wave-uniform
+    const bool valid = lane < n;
+    const uint32_t my_v = valid ? edge_col(g, e + lane) : u;
+    const uint32_t my_w = valid ? edge_w(g, e + lane) : 0u;
     const uint64_t fm = front[u];
+    const uint32_t du_l = dist[(uint64_t)u * 64 + lane];  // lane l: query l's distance at u
     if (__popcll(fm) <= SPARSE_LANES) {
-      // few queries active at u: lanes take CHUNK different edges at once
-      // and loop over the active query bits (wave-uniform branch)
-      const uint64_t ee = e + lane;
-      const bool valid = ee < e1;
-      const uint32_t v = valid ? edge_col(g, ee) : 0u;
-      const uint32_t w = valid ? edge_w(g, ee) : 0u;
+      // few queries active at u: lanes take different edges and loop over
+      // the active query bits (wave-uniform loop)
       uint64_t imp = 0;
       for (uint64_t m = fm; m; m &= m - 1) {
         const uint32_t qb = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-        const uint32_t du = dist[(uint64_t)u * 64 + qb];
-        const uint64_t nd = (uint64_t)du + w;
-        if (valid && du < INFINIT && nd < INFINIT) {
-          uint32_t *dv = &dist[(uint64_t)v * 64 + qb];
-          if (nd < *dv && atomicMin(dv, (uint32_t)nd) > nd) imp |= 1ull << qb;
+        const uint32_t du = __builtin_amdgcn_readlane(du_l, qb);
+        const uint64_t nd = (uint64_t)du + my_w;
+        uint32_t *dv = &dist[(uint64_t)my_v * 64 + qb];
+        if (valid && nd < INFINIT && nd < *dv) {
+          atomicMin(dv, (uint32_t)nd);
+          imp |= 1ull << qb;
         }
       }
-      if (imp && (next[v] & imp) != imp) atomicOr((unsigned long long *)&next[v], imp);
+      if (imp) atomicOr((unsigned long long *)&next[my_v], imp);
       continue;
     }
     const bool act = (fm >> lane) & 1ull;
-    const uint32_t du = act ? dist[(uint64_t)u * 64 + lane] : INFINIT;
-    for (; e < e1; e++) {
-      uint32_t v = edge_col(g, e);
-      uint64_t nd = (uint64_t)du + edge_w(g, e);
-      bool imp = false;
-      // distances only decrease and next[] bits are only set within this
-      // kernel: a plain (possibly stale) read that already shows no
-      // improvement, or the bits set, makes the device-scope atomic moot
-      uint32_t *dv = &dist[(uint64_t)v * 64 + lane];
-      if (act && nd < INFINIT && nd < *dv) imp = atomicMin(dv, (uint32_t)nd) > nd;
-      uint64_t m = __ballot(imp);
-      if (m && lane == 0 && (next[v] & m) != m) atomicOr((unsigned long long *)&next[v], m);
+    const uint32_t du = act ? du_l : INFINIT;
+    constexpr uint32_t U = 8;
+    for (uint32_t k0 = 0; k0 < n; k0 += U) {
+      uint32_t vv[U], dd[U];
+#pragma unroll
+      for (uint32_t k = 0; k < U; k++) {  // issue up to U row reads
+        const uint32_t jj = k0 + k < n ? k0 + k : n - 1;
+        vv[k] = __builtin_amdgcn_readlane(my_v, jj);
+        dd[k] = dist[(uint64_t)vv[k] * 64 + lane];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < U; k++) {
+        if (k0 + k >= n) break;
+        const uint64_t nd = (uint64_t)du + __builtin_amdgcn_readlane(my_w, k0 + k);
+        const bool imp = nd < INFINIT && nd < dd[k];
+        if (imp) atomicMin(&dist[(uint64_t)vv[k] * 64 + lane], (uint32_t)nd);
+        const uint64_t m = __ballot(imp);
+        if (m && lane == 0) atomicOr((unsigned long long *)&next[vv[k]], m);
+      }
     }
   }
 }
@@ -319,7 +342,31 @@ __global__ void k_gather_col(const uint32_t *dist, uint64_t V, uint32_t lane, ui
   for (; i < V; i += stride) row[i] = dist[i * 64 + lane];
 }
 
-__global__ void k_digest(const uint32_t *row, uint64_t V, unsigned long long *out) {
+// the block's sums land in out[0], out[1] with one atomic pair per block
+// (atomics from every wave on one address serialise at the L2)
+__device__ __forceinline__ void block_add2(unsigned long long h, unsigned long long n,
+                                           unsigned long long *out) {
+  __shared__ unsigned long long s_h[BLOCK / WAVE], s_n[BLOCK / WAVE];
+  h = wave_sum(h);
+  n = wave_sum(n);
+  const uint32_t wid = threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    s_h[wid] = h;
+    s_n[wid] = n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < BLOCK / WAVE; w++) {
+      h += s_h[w];
+      n += s_n[w];
+    }
+    atomicAdd(&out[0], h);
+    atomicAdd(&out[1], n);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_digest(const uint32_t *row, uint64_t V,
+                                                  unsigned long long *out) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long h = 0, n = 0;
@@ -328,21 +375,16 @@ __global__ void k_digest(const uint32_t *row, uint64_t V, unsigned long long *ou
     h += digest_term(i, x);
     n += (x != INFINIT);
   }
-  h = wave_sum(h);
-  n = wave_sum(n);
-  if (lane_id() == 0) {
-    atomicAdd(&out[0], h);
-    atomicAdd(&out[1], n);
-  }
+  block_add2(h, n, out);
 }
 
 // digests of the finished BFS lanes (rows of the row-major [lanes][V]
 // state): blockIdx.y picks the k-th lane of the mask
-__global__ void k_digest_rows(const uint32_t *vals, uint64_t V, uint64_t mask,
-                              unsigned long long *out) {
+__global__ __launch_bounds__(BLOCK) void k_digest_rows(const uint32_t *vals, uint64_t V,
+                                                       uint64_t mask, unsigned long long *out) {
   uint64_t m = mask;
   for (uint32_t k = 0; k < blockIdx.y; k++) m &= m - 1;  // drop the lowest set bits
-  if (!m) return;
+  if (!m) return;  // block-uniform
   const uint32_t lane = (uint32_t)(__ffsll((unsigned long long)m) - 1);
   const uint32_t *row = vals + (uint64_t)lane * V;
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -353,12 +395,7 @@ __global__ void k_digest_rows(const uint32_t *vals, uint64_t V, uint64_t mask,
     h += digest_term(i, x);
     n += (x != INFINIT);
   }
-  h = wave_sum(h);
-  n = wave_sum(n);
-  if (lane_id() == 0) {
-    atomicAdd(&out[2 * lane], h);
-    atomicAdd(&out[2 * lane + 1], n);
-  }
+  block_add2(h, n, out + 2 * lane);
 }
 
 // digests of the finished SSSP lanes in one sweep of the vertex-major
@@ -399,7 +436,7 @@ void row_digest(Context &ctx, const uint32_t *row, uint64_t V, uint64_t *reached
                 uint64_t *checksum) {
   auto *d = reinterpret_cast<unsigned long long *>(ctx.d_ctl.p) + 48;
   GRUS_HIP(hipMemsetAsync(d, 0, 16, ctx.stream));
-  ms::k_digest<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(row, V, d);
+  ms::k_digest<<<grid_cap(ctx, V, BLOCK, 4), BLOCK, 0, ctx.stream>>>(row, V, d);
   GRUS_LAUNCH_CHECK();
   ctx.read_ctl(50);
   *checksum = ctx.h_ctl[48];
@@ -632,7 +669,7 @@ void MsEngine::extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t
   if (reached || checksum) {
     auto *d = digest_.as<unsigned long long>();
     GRUS_HIP(hipMemsetAsync(d, 0, 16, ctx.stream));
-    ms::k_digest<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(row, V, d);
+    ms::k_digest<<<grid_cap(ctx, V, BLOCK, 4), BLOCK, 0, ctx.stream>>>(row, V, d);
     GRUS_LAUNCH_CHECK();
     unsigned long long h[2];
     GRUS_HIP(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, ctx.stream));
