@@ -219,6 +219,12 @@ __global__ void k_chunk_counts(const uint64_t *off, const uint32_t *q, uint32_t 
   }
 }
 
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, uint32_t i) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, i);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), i);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Relaxation of one 64-edge chunk of frontier vertex u per wave task.  Lane l
 // loads edge e+l of the chunk (coalesced), and the edges are then broadcast to
 // the wave with readlane.  Distances only decrease within a round, so a plain
@@ -233,24 +239,45 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t 
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
   const uint32_t lane = lane_id();
   // each wave takes a contiguous run of chunks (equal counts, so equal work
-  // within one chunk), finds the owner of its first chunk by binary search
-  // and then walks the chunk scan forward
+  // within one chunk).  The queue entries its chunks belong to are read 64 at
+  // a time, one per lane (scan value, vertex, row bounds, front word): the
+  // owner of chunk c is then the highest lane whose scan value is <= c (a
+  // ballot), and the entry's fields are broadcast with readlane, so the
+  // per-chunk chain of dependent loads is gone
   const uint64_t per = (chunks + nwaves - 1) / nwaves;
   const uint64_t c0 = wave * per;
   const uint64_t c1 = c0 + per < chunks ? c0 + per : chunks;
   if (c0 >= c1) return;
-  uint32_t j = upper_bound_u64(CS, 0, F + 1, c0) - 1;
+  uint32_t jb = upper_bound_u64(CS, 0, F + 1, c0) - 1;  // owner of c0
+  uint64_t w_cs = 0, w_off = 0, w_end = 0, w_fm = 0, wend = 0;
+  uint32_t w_u = 0;
+  auto load_window = [&]() {  // queue entries jb .. jb+63
+    const uint64_t jl = (uint64_t)jb + lane;
+    w_cs = jl <= F ? CS[jl] : ~0ull;  // CS[F] = chunks: never an owner
+    const bool in = jl < F;
+    w_u = in ? q[jl] : 0u;
+    w_off = in ? g.off[w_u] : 0ull;
+    w_end = in ? g.off[w_u + 1] : 0ull;
+    w_fm = in ? front[w_u] : 0ull;
+    wend = (uint64_t)jb + WAVE <= F ? CS[jb + WAVE] : chunks;  // first chunk past the window
+  };
+  load_window();
   for (uint64_t c = c0; c < c1; c++) {
-    while (CS[j + 1] <= c) j++;  // skips queued vertices without out-edges
-    const uint32_t u = q[j];
-    const uint64_t e = g.off[u] + (c - CS[j]) * CHUNK;
-    uint64_t e1 = g.off[u + 1];
+    while (c >= wend) {  // (queued vertices without out-edges own no chunk)
+      jb += WAVE;
+      load_window();
+    }
+    const uint64_t below = __ballot(w_cs <= c);  // never 0: lane 0's scan value is <= c
+    const uint32_t l = 63u - (uint32_t)__clzll((long long)below);
+    const uint32_t u = __builtin_amdgcn_readlane(w_u, l);
+    const uint64_t e = readlane_u64(w_off, l) + (c - readlane_u64(w_cs, l)) * CHUNK;
+    uint64_t e1 = readlane_u64(w_end, l);
     if (e1 > e + CHUNK) e1 = e + CHUNK;
     const uint32_t n = (uint32_t)(e1 - e);  // 1..64, wave-uniform
     const bool valid = lane < n;
     const uint32_t my_v = valid ? edge_col(g, e + lane) : u;
     const uint32_t my_w = valid ? edge_w(g, e + lane) : 0u;
-    const uint64_t fm = front[u];
+    const uint64_t fm = readlane_u64(w_fm, l);
     const uint32_t du_l = dist[(uint64_t)u * 64 + lane];  // lane l: query l's distance at u
     if (__popcll(fm) <= SPARSE_LANES) {
       // few queries active at u: lanes take different edges and loop over
