@@ -419,13 +419,13 @@ def run_serving(grus):
         assert all(r.status == 0 for r in out)
         return N / tb
 
-    # the library's default policy: one engine per kind, a batch admitted
+    # the library's default policy: two engines per kind, a batch admitted
     # into an idle engine; and lane-by-lane admission into 8 engines per kind
     qps = saturate()
     qps_cont = saturate(max_inflight_batches=8, admit_min_free=1)
     res["batched_qps"] = round(qps, 1)
     res["speedup_vs_serial"] = round(qps / serial_qps, 2)
-    res["policy"] = "1 engine per kind x 64 lanes, admit into idle engines"
+    res["policy"] = "2 engines per kind x 64 lanes, admit into idle engines"
     res["continuous_8_engines_qps"] = round(qps_cont, 1)
     sweep = []
     total = N * 2
