@@ -42,7 +42,8 @@ class Tuning(C.Structure):
                 ("bu_thread_edges", C.c_uint32), ("sssp_delta", C.c_uint32),
                 ("pr_layout", C.c_uint32), ("pr_bin_bits", C.c_uint32),
                 ("sssp_pull_div", C.c_uint32), ("sssp_local_hops", C.c_uint32),
-                ("pr_task_edges", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("pr_task_edges", C.c_uint32), ("sssp_pull_layout", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class GraphInfo(C.Structure):

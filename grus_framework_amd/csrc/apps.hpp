@@ -17,6 +17,26 @@ void run_cc(Graph &g, int method, grus_stats *st);
 void run_bfs_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);
 void run_sssp_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);
 
+// one SSSP pull round over the binned push layout (pagerank.hip): every
+// out-edge (u, v) of a frontier vertex u offers dist[u] + w to v, and v
+// takes the minimum when it is below dist[v]; improved vertices join the
+// near queue (below T_thr) or the far list (once per far epoch), as in the
+// window pull round of sssp.hip
+struct StepCounters;
+struct BinnedMinArgs {
+  const uint64_t *front;  // frontier bitmap
+  uint32_t *dist;
+  const uint64_t *out_off;
+  uint32_t T_thr, near_id, far_id;
+  uint32_t *near_stamp, *far_stamp;
+  uint32_t *qn, *qfar;  // near queue; far list at its current end
+  StepCounters *cnt;    // next_count, next_edges, far_count
+};
+// builds the binned layout, its tasks and its weight stream when missing;
+// false where the layout does not apply (then pull rounds use the window)
+bool sssp_binned_ready(Graph &g);
+void sssp_binned_round(Graph &g, const BinnedMinArgs &a);
+
 // autotuner (tune.cpp)
 void autotune(Graph &g, int kind, const uint32_t *srcs, int n, grus_tuning *out);
 uint64_t graph_signature(const Graph &g);

@@ -121,6 +121,7 @@ int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in) {
     if (in->sssp_local_hops > 256) mgg::raise(GRUS_E_INVALID, "sssp_local_hops above 256");
     if (in->pr_task_edges != 0 && in->pr_task_edges < 256)
       mgg::raise(GRUS_E_INVALID, "pr_task_edges below 256");
+    if (in->sssp_pull_layout > 2) mgg::raise(GRUS_E_INVALID, "sssp_pull_layout");
     ctx->c->tuning = *in;
   });
 }

@@ -94,6 +94,12 @@ struct PrWork {
   const void *bp_for = nullptr;
   std::vector<uint32_t> bp_bstart;  // first phase-2 edge of every bin (host)
   uint64_t bp_task_edges = 0;       // task size the task table was cut with
+  // SSSP pull rounds over the same layout: w2 = edge weights in phase-2
+  // stream order (slab-transposed like d2); gmin = minima of the bins split
+  // over several tasks (all 0xFFFFFFFF between rounds)
+  Buf bp_w2, bp_gmin;
+  const void *bp_w2_for = nullptr;
+  const void *bp_unusable_for = nullptr;  // offsets a failed build was tried on
 };
 
 __global__ void k_window_rows(const uint64_t *off, uint64_t V, uint64_t nwin, uint32_t *win_row) {
@@ -325,13 +331,31 @@ __device__ __forceinline__ float from_fixed(unsigned long long s) {
 // RMAT s24).  The chunk's contributions (at most BP_SPAN sources) and run
 // offsets are staged in LDS first: read straight from memory, the stream's
 // source order made each wave's loads touch dozens of lines.
+//
+// The value of a source is any 4-byte pattern Src gives: PageRank's
+// contribution, or (SSSP pull rounds) the distance of a frontier vertex.
+struct ContribValue {
+  const float *contrib;
+  __device__ __forceinline__ uint32_t operator()(uint32_t u) const {
+    return __float_as_uint(contrib[u]);
+  }
+};
+struct FrontDistValue {  // 0xFFFFFFFF: not in the frontier
+  const uint64_t *front;
+  const uint32_t *dist;
+  __device__ __forceinline__ uint32_t operator()(uint32_t u) const {
+    const uint32_t d = dist[u];  // both loads issued before either is used
+    return bm_test(front, u) ? d : 0xFFFFFFFFu;
+  }
+};
+
+template <class Src>
 __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, const uint32_t *ch_i0,
                                                       const uint32_t *ch_r0, const uint16_t *p1,
                                                       const uint32_t *rbase, const uint32_t *rtab,
-                                                      uint32_t nbins, const float *contrib,
-                                                      float *val) {
+                                                      uint32_t nbins, Src src, uint32_t *val) {
   extern __shared__ uint32_t sr[];  // the chunk's run offsets (<= nbins), then BP_SPAN values
-  float *sc = reinterpret_cast<float *>(sr + nbins);
+  uint32_t *sc = sr + nbins;
   const uint64_t c = blockIdx.x;
   const uint32_t u0 = ch_u0[c], span = ch_u0[c + 1] - u0;  // span >= 1
   const uint32_t r0 = ch_r0[c], nr = ch_r0[c + 1] - r0;    // nr <= nbins
@@ -352,11 +376,11 @@ __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, con
     }
   }
   for (uint32_t jb = 0; jb < span; jb += BLOCK * BP_UNROLL) {
-    float x[BP_UNROLL];
+    uint32_t x[BP_UNROLL];
 #pragma unroll
     for (int q = 0; q < BP_UNROLL; q++) {
       const uint32_t j = jb + threadIdx.x + BLOCK * q;
-      x[q] = contrib[u0 + (j < span ? j : span - 1u)];
+      x[q] = src(u0 + (j < span ? j : span - 1u));
     }
 #pragma unroll
     for (int q = 0; q < BP_UNROLL; q++) {
@@ -573,6 +597,217 @@ __global__ __launch_bounds__(BLOCK) void k_bp_finish(const uint32_t *mbins, int 
   const unsigned long long s = gacc[v];
   gacc[v] = 0ull;
   finish<kLast>(v, from_fixed(s), out_off, next, base, alpha);
+}
+
+// ---- SSSP pull rounds over the binned layout ----
+// The finish of one bin's rows (v0 + i, i < nb), BP_BLOCK threads with the
+// rows i = tid + BP_BLOCK r: v takes best(i) when it is below dist[v], then
+// joins the near queue (below the threshold; its out-degree counts toward
+// the next round's edges) or the far list (once per far epoch).  Only the
+// finisher of v writes dist[v], so distances need no atomics.  The workgroup
+// appends both lists with one global atomic each (a block scan of the
+// per-thread counts): an atomic per wave on the two counters serialised
+// ~10^5 waves per round.
+constexpr int BPR = (1 << BP_MAX_BITS) / BP_BLOCK;  // rows per thread at most
+static_assert(BPR <= 32, "row bits per thread");
+
+template <class Best>
+__device__ __forceinline__ void finish_rows(uint64_t v0, uint32_t nb, Best best_of,
+                                            const BinnedMinArgs &a) {
+  constexpr uint32_t NW = BP_BLOCK / WAVE;
+  __shared__ uint32_t s_w[NW];
+  __shared__ uint32_t s_base[2];
+  __shared__ unsigned long long s_e[NW];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  uint32_t nearm = 0, farm = 0;
+  uint64_t edges = 0;
+#pragma unroll
+  for (int r = 0; r < BPR; r++) {
+    const uint32_t i = threadIdx.x + BP_BLOCK * r;
+    if (i >= nb) continue;
+    const uint32_t best = best_of(i);
+    const uint64_t v = v0 + i;
+    if (best < INFINIT && best < a.dist[v]) {
+      a.dist[v] = best;
+      if (best < a.T_thr) {
+        a.near_stamp[v] = a.near_id;
+        nearm |= 1u << r;
+        edges += a.out_off[v + 1] - a.out_off[v];
+      } else if (a.far_stamp[v] != a.far_id) {
+        a.far_stamp[v] = a.far_id;
+        farm |= 1u << r;
+      }
+    }
+  }
+  // block exclusive scan of (near count << 16 | far count): both stay below
+  // 2^15 per workgroup, so the halves never carry into each other
+  const uint32_t mine = ((uint32_t)__popc(nearm) << 16) | (uint32_t)__popc(farm);
+  uint32_t inc = mine;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o);
+    if (lane >= (uint32_t)o) inc += y;
+  }
+  const unsigned long long ew = wave_sum((unsigned long long)edges);
+  if (lane == WAVE - 1) s_w[wid] = inc;
+  if (lane == 0) s_e[wid] = ew;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  unsigned long long etot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < NW; w++) {
+    before += w < wid ? s_w[w] : 0u;
+    total += s_w[w];
+    etot += s_e[w];
+  }
+  if (threadIdx.x == 0) {
+    s_base[0] = (total >> 16) ? atomicAdd(&a.cnt->next_count, total >> 16) : 0u;
+    s_base[1] = (total & 0xFFFFu) ? atomicAdd(&a.cnt->far_count, total & 0xFFFFu) : 0u;
+    if (etot) atomicAdd((unsigned long long *)&a.cnt->next_edges, etot);
+  }
+  __syncthreads();
+  const uint32_t excl = before + inc - mine;
+  uint32_t pn = s_base[0] + (excl >> 16), pf = s_base[1] + (excl & 0xFFFFu);
+  for (uint32_t m = nearm; m; m &= m - 1)
+    a.qn[pn++] = (uint32_t)(v0 + threadIdx.x + BP_BLOCK * (uint32_t)__ffs(m) - BP_BLOCK);
+  for (uint32_t m = farm; m; m &= m - 1)
+    a.qfar[pf++] = (uint32_t)(v0 + threadIdx.x + BP_BLOCK * (uint32_t)__ffs(m) - BP_BLOCK);
+}
+
+// Phase 2 of an SSSP pull round: the phase-1 values are the distances of
+// frontier sources (0xFFFFFFFF for the others); entry e of a bin's stream
+// offers val[group(e)] + w2[e] to its destination, and the minimum is kept
+// in LDS (exact and order-independent).  The stream walk is k_bp_accum's;
+// w2 is stored like d2, so lane l's 32 bytes of a slab are the weights of
+// its entries l, l + 64, ...  A bin's only task finishes its rows; pieces of
+// split bins take the minimum into gmin, which k_bp_min_finish completes.
+__global__ __launch_bounds__(BP_BLOCK) void k_bp_min(const BpTask *tasks, const uint16_t *d2,
+                                                     const uint32_t *w2, const uint32_t *gs512,
+                                                     const uint32_t *val, uint64_t V, int bits,
+                                                     BinnedMinArgs a, uint32_t *gmin) {
+  __shared__ uint32_t acc[1 << BP_MAX_BITS];
+  __shared__ uint32_t win[BP_BLOCK / WAVE][BP_SLAB];
+  const BpTask t = tasks[blockIdx.x];
+  const uint64_t v0 = (uint64_t)t.bin << bits;
+  const uint32_t nb = (uint32_t)((V - v0) < (1ull << bits) ? (V - v0) : (1ull << bits));
+  for (uint32_t i = threadIdx.x; i < nb; i += BP_BLOCK) acc[i] = 0xFFFFFFFFu;
+  __syncthreads();
+  constexpr uint32_t NW = BP_BLOCK / WAVE;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  uint32_t *wv = win[wid];
+  const uint32_t s1 = (uint32_t)((t.e1 + (uint64_t)BP_SLAB - 1) / BP_SLAB);
+  uint32_t s = t.e0 / BP_SLAB + wid;
+  u32x4 nx = {0, 0, 0, 0}, nw0 = {0, 0, 0, 0}, nw1 = {0, 0, 0, 0};
+  uint32_t ngs = 0;
+  if (s < s1) {
+    const uint64_t o = (uint64_t)s * BP_SLAB + lane * BP_EPT;
+    nx = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(d2 + o));
+    nw0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o));
+    nw1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o) + 1);
+    ngs = gs512[s];
+  }
+  for (; s < s1; s += NW) {
+    const u32x4 cur = nx, cw0 = nw0, cw1 = nw1;
+    const uint32_t gbase = ngs;
+    const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    const uint32_t wt[BP_EPT] = {cw0.x, cw0.y, cw0.z, cw0.w, cw1.x, cw1.y, cw1.z, cw1.w};
+    uint32_t ent[BP_EPT];
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) ent[q] = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
+    uint32_t gq[BP_EPT];
+    uint32_t run = gbase - 1u;
+    uint64_t b0 = 0;
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) {
+      const uint64_t bq = __ballot(ent[q] >> 15);
+      if (q == 0) b0 = bq;
+      gq[q] = run + lane_rank(bq) + (ent[q] >> 15);
+      run += (uint32_t)__popcll(bq);
+    }
+    const uint32_t w0 = (b0 & 1ull) ? gbase : gbase - 1u;
+    uint32_t wl[BP_SLAB / WAVE];
+#pragma unroll
+    for (int j = 0; j < BP_SLAB / WAVE; j++) wl[j] = val[w0 + lane + WAVE * j];
+    {
+      const uint32_t sn = s + NW < s1 ? s + NW : s1 - 1u;
+      const uint64_t o = (uint64_t)sn * BP_SLAB + lane * BP_EPT;
+      nx = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(d2 + o));
+      nw0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o));
+      nw1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o) + 1);
+      ngs = gs512[sn];
+    }
+#pragma unroll
+    for (int j = 0; j < BP_SLAB / WAVE; j++) wv[lane + WAVE * j] = wl[j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t x[BP_EPT];
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) x[q] = wv[gq[q] - w0];
+    const uint32_t sb = s * BP_SLAB;
+    const bool whole = sb >= t.e0 && sb + BP_SLAB <= t.e1;  // wave-uniform
+#pragma unroll
+    for (int q = 0; q < BP_EPT; q++) {
+      const uint64_t c = (uint64_t)x[q] + wt[q];
+      const uint32_t pe = sb + WAVE * q + lane;
+      const bool mine = whole || (pe >= t.e0 && pe < t.e1);
+      if (mine && x[q] < INFINIT && c < INFINIT)
+        atomicMin(&acc[ent[q] & ((1u << BP_MAX_BITS) - 1u)], (uint32_t)c);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __syncthreads();
+  if (!t.single) {  // block-uniform
+#pragma unroll
+    for (int r = 0; r < BPR; r++) {
+      const uint32_t i = threadIdx.x + BP_BLOCK * r;
+      if (i < nb && acc[i] != 0xFFFFFFFFu) atomicMin(&gmin[v0 + i], acc[i]);
+    }
+    return;
+  }
+  finish_rows(v0, nb, [&](uint32_t i) { return acc[i]; }, a);
+}
+
+// rows of the bins split over several tasks, one workgroup per bin; resets
+// gmin for the next round
+__global__ __launch_bounds__(BP_BLOCK) void k_bp_min_finish(const uint32_t *mbins, int bits,
+                                                            uint64_t V, uint32_t *gmin,
+                                                            BinnedMinArgs a) {
+  const uint64_t v0 = (uint64_t)mbins[blockIdx.x] << bits;
+  const uint32_t nb = (uint32_t)((V - v0) < (1ull << bits) ? (V - v0) : (1ull << bits));
+  finish_rows(v0, nb, [&](uint32_t i) {
+    const uint32_t best = gmin[v0 + i];
+    if (best != 0xFFFFFFFFu) gmin[v0 + i] = 0xFFFFFFFFu;
+    return best;
+  }, a);
+}
+
+// weight stream keys: CSR edge e -> (bin of its destination, weight); the
+// same keys as the d2 sort, so the same stable sort puts the weights in d2's
+// order
+__global__ void k_bp_w2_keys(const uint64_t *off, const uint32_t *col, const uint32_t *w,
+                             uint64_t V, int bits, uint64_t *key, uint32_t *wt) {
+  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  for (uint64_t u = wave; u < V; u += nw)
+    for (uint64_t e = off[u] + lane_id(); e < off[u + 1]; e += WAVE) {
+      key[e] = col[e] >> bits;
+      wt[e] = w[e];
+    }
+}
+
+// w2: the sorted weights, padded with zeros to a multiple of BP_SLAB, each
+// slab stored transposed as d2
+__global__ void k_bp_w2(const uint32_t *wt, uint64_t E, uint64_t Epad, uint32_t *w2) {
+  uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; p < Epad; p += stride) {
+    const uint64_t r = p % BP_SLAB;
+    const uint64_t phys = p - r + (r % WAVE) * BP_EPT + r / WAVE;
+    w2[phys] = p < E ? wt[p] : 0u;
+  }
 }
 
 // ---- binned push layout construction ----
@@ -907,9 +1142,11 @@ bool build_binned(Graph &g, PrWork &w, int bits) {
   GRUS_HIP(hipMemGetInfo(&free_b, &total_b));
   if (free_b < 48 * E + 16 * V + (1ull << 30)) return false;
   for (Buf *b : {&w.bp_p1, &w.bp_ch_u0, &w.bp_ch_i0, &w.bp_ch_r0, &w.bp_rtab, &w.bp_rbase,
-                 &w.bp_val, &w.bp_d2, &w.bp_gs512, &w.bp_task, &w.bp_multi, &w.bp_gacc})
+                 &w.bp_val, &w.bp_d2, &w.bp_gs512, &w.bp_task, &w.bp_multi, &w.bp_gacc,
+                 &w.bp_w2})
     b->release();
   w.bp_for = nullptr;
+  w.bp_w2_for = nullptr;
   const unsigned rows_grid = grid_cap(ctx, V * WAVE, BLOCK);
   Buf flag, excl;
   flag.alloc(E * 4);
@@ -1142,7 +1379,7 @@ void run_binned(Graph &g, PrWork &w, float alpha, int iters, grus_stats *st) {
     pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
         w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
         w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
-        contrib, w.bp_val.as<float>());
+        pr::ContribValue{contrib}, w.bp_val.as<uint32_t>());
     GRUS_LAUNCH_CHECK();
     if (last) {
       pr::k_bp_accum<true><<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
@@ -1177,7 +1414,91 @@ void run_binned(Graph &g, PrWork &w, float alpha, int iters, grus_stats *st) {
     st->kernel_launches = launches;
   }
 }
+// the weight stream of the binned layout (for SSSP pull rounds): the CSR
+// weights sorted stably by destination bin, as d2 was, so w2 lines up with
+// d2 entry by entry
+bool build_binned_weights(Graph &g, PrWork &w) {
+  Context &ctx = *g.ctx;
+  const CsrView out = g.csr();
+  const uint64_t V = g.V, E = g.E;
+  if (!out.w || out.split < E) return false;
+  size_t free_b = 0, total_b = 0;
+  GRUS_HIP(hipMemGetInfo(&free_b, &total_b));
+  const uint64_t Epad = div_up(E, pr::BP_SLAB) * pr::BP_SLAB;
+  if (free_b < 24 * E + 4 * Epad + (1ull << 30)) return false;
+  w.bp_w2.release();
+  {
+    Buf ka, kb, va, vb;
+    ka.alloc(E * 8);
+    kb.alloc(E * 8);
+    va.alloc(E * 4);
+    vb.alloc(E * 4);
+    pr::k_bp_w2_keys<<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
+        out.off, out.col, out.w, V, w.bp_bits, ka.as<uint64_t>(), va.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    uint32_t *wt = nullptr;
+    sort_pairs_u64_u32(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<uint32_t>(),
+                       vb.as<uint32_t>(), E, &wt, std::max(1, bits_for(w.bp_nbins - 1)));
+    w.bp_w2.alloc(Epad * 4);
+    pr::k_bp_w2<<<grid_cap(ctx, Epad, BLOCK), BLOCK, 0, ctx.stream>>>(wt, E, Epad,
+                                                                     w.bp_w2.as<uint32_t>());
+    GRUS_LAUNCH_CHECK();
+    ctx.sync();  // the sort buffers are freed on scope exit
+  }
+  w.bp_w2_for = out.off;
+  return true;
+}
 }  // namespace
+
+bool sssp_binned_ready(Graph &g) {
+  if (g.placement != GRUS_PLACE_HBM || g.V == 0) return false;
+  Context &ctx = *g.ctx;
+  const CsrView out = g.csr();
+  if (!out.w) return false;
+  if (!g.pr) g.pr = std::make_shared<PrWork>();
+  PrWork &w = *g.pr;
+  if (w.bp_unusable_for == out.off) return false;
+  if (w.bp_for != out.off) {  // a layout PageRank already built is used as it is
+    const int bits = std::min(ctx.tuning.pr_bin_bits ? (int)ctx.tuning.pr_bin_bits : PB_BITS,
+                              pr::BP_MAX_BITS);
+    if (!build_binned(g, w, bits)) {
+      w.bp_unusable_for = out.off;
+      return false;
+    }
+  }
+  if (w.bp_task_edges == 0)
+    build_binned_tasks(g, w, ctx.tuning.pr_task_edges ? ctx.tuning.pr_task_edges : BP_TASK_EDGES);
+  if (w.bp_w2_for != out.off && !build_binned_weights(g, w)) {
+    w.bp_unusable_for = out.off;
+    return false;
+  }
+  if (w.bp_gmin.bytes < g.V * 4) {
+    w.bp_gmin.alloc(g.V * 4);
+    fill_u32(ctx, w.bp_gmin.as<uint32_t>(), g.V, 0xFFFFFFFFu);
+  }
+  return true;
+}
+
+void sssp_binned_round(Graph &g, const BinnedMinArgs &a) {
+  Context &ctx = *g.ctx;
+  PrWork &w = *g.pr;
+  const int bits = w.bp_bits;
+  pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
+      w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
+      w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
+      pr::FrontDistValue{a.front, a.dist}, w.bp_val.as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  pr::k_bp_min<<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
+      w.bp_task.as<const pr::BpTask>(), w.bp_d2.as<uint16_t>(), w.bp_w2.as<uint32_t>(),
+      w.bp_gs512.as<uint32_t>(), w.bp_val.as<uint32_t>(), g.V, bits, a,
+      w.bp_gmin.as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  if (w.bp_nmulti) {
+    pr::k_bp_min_finish<<<(unsigned)w.bp_nmulti, pr::BP_BLOCK, 0, ctx.stream>>>(
+        w.bp_multi.as<uint32_t>(), bits, g.V, w.bp_gmin.as<uint32_t>(), a);
+    GRUS_LAUNCH_CHECK();
+  }
+}
 
 void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
   Context &ctx = *g.ctx;

@@ -578,6 +578,16 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   // (not under a CU mask: a cooperative grid must be co-resident, and the
   // runtime's residency check does not know the mask)
   const bool persistent = g.max_deg <= 32 && !no_persist && !ctx.cu_masked;
+  // a round pulls when its frontier's out-edges exceed E / pull_div
+  // (automatic: 16, measured best of 4..128 on RMAT s24; 0xFFFFFFFF: never)
+  const uint32_t pull_div = ctx.tuning.sssp_pull_div ? ctx.tuning.sssp_pull_div : 16;
+  const uint64_t pull_edges = pull_div == 0xFFFFFFFFu ? ~0ull : g.E / pull_div;
+  const bool can_pull = g.symmetric || g.placement != GRUS_PLACE_CAPPED;  // needs the in-edges
+  // pull rounds: binned push where the layout applies (built on first use
+  // and kept with the graph), else the window reduction over the in-edges
+  const uint32_t pull_layout = ctx.tuning.sssp_pull_layout;
+  const bool binned_pull = pull_edges != ~0ull && pull_layout != 1 && !persistent &&
+                           g.E > 0 && sssp_binned_ready(g);
   GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   sssp::k_init<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(dist, V, src,
                                                                  w.q[0].as<uint32_t>());
@@ -592,11 +602,6 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   uint64_t T = delta;
   uint32_t far_id = w.next_stamp(ctx, V);
   const uint64_t round_cap = 64 * V + 1024;
-  // a round pulls when its frontier's out-edges exceed E / pull_div
-  // (automatic: 16, measured best of 4..128 on RMAT s24; 0xFFFFFFFF: never)
-  const uint32_t pull_div = ctx.tuning.sssp_pull_div ? ctx.tuning.sssp_pull_div : 16;
-  const uint64_t pull_edges = pull_div == 0xFFFFFFFFu ? ~0ull : g.E / pull_div;
-  const bool can_pull = g.symmetric || g.placement != GRUS_PLACE_CAPPED;  // needs the in-edges
   const uint64_t W = V / 64 + 1;
   for (;;) {
     while (nn) {
@@ -605,7 +610,28 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
       GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
       const uint32_t Tk = (uint32_t)std::min<uint64_t>(T, 0xFFFFFFFFull);
       const uint32_t near_id = w.next_stamp(ctx, V);
-      if (can_pull && mf > pull_edges) {
+      if (binned_pull && mf > pull_edges) {
+        // push over the binned layout: sequential streams instead of the
+        // window's gathers of dist[u] (one cache line per in-edge)
+        uint64_t *fbm = w.front_bm.as<uint64_t>();
+        fill_u64(ctx, fbm, W, 0);
+        queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nn, fbm, W);
+        BinnedMinArgs a{};
+        a.front = fbm;
+        a.dist = dist;
+        a.out_off = out.off;
+        a.T_thr = Tk;
+        a.near_id = near_id;
+        a.far_id = far_id;
+        a.near_stamp = w.near_stamp.as<uint32_t>();
+        a.far_stamp = w.far_stamp.as<uint32_t>();
+        a.qn = w.q[cur ^ 1].as<uint32_t>();
+        a.qfar = w.far[fcur].as<uint32_t>() + nfar;
+        a.cnt = cnt;
+        sssp_binned_round(g, a);
+        launches += 5;
+        pull_rounds++;
+      } else if (can_pull && mf > pull_edges) {
         const CsrView in = g.csc();
         if (w.win_for != in.off) {
           w.nwin = in.E / WIN + 1;

@@ -119,7 +119,14 @@ typedef struct grus_tuning {
   uint32_t pr_task_edges; /* binned PageRank layout: edges per accumulation
                              task; a bin with more edges is split over several
                              tasks (0 = automatic, 2^18) */
-  uint32_t reserved[3];
+  uint32_t sssp_pull_layout; /* SSSP pull rounds: 0 automatic (= 2 where it
+                             applies, else 1); 1 window reduction over the
+                             in-edges; 2 binned push over PageRank's binned
+                             layout plus a weight stream in its order
+                             (HBM-resident weighted graphs that layout 4 of
+                             pr_layout applies to, else 1).  Distances are
+                             identical either way. */
+  uint32_t reserved[2];
 } grus_tuning;
 
 /* ---- autotuning (no reference counterpart: the reference hard-codes its

@@ -396,11 +396,12 @@ def test_pagerank_binned_falls_back_to_pull(ctx):
         assert sa.kernel_launches == sb.kernel_launches
 
 
-def sssp_with(ctx, g, src, pull_div):
+def sssp_with(ctx, g, src, pull_div, layout=0):
     t0 = ctx.tuning
     try:
         t = ctx.tuning
         t.sssp_pull_div = pull_div
+        t.sssp_pull_layout = layout
         ctx.tuning = t
         return grus.sssp(g, src)
     finally:
@@ -420,15 +421,52 @@ def test_sssp_pull_rounds_match_push(ctx, oracle):
             grus.Graph.from_csr(ctx, off, col, w, symmetric=False) as d:
         for gr, src in ((g, g.info.max_degree_vertex), (d, 7)):
             ref = oracle.sssp(host_csr(gr), src)
-            for div in (ALWAYS_PULL, NEVER_PULL, 32):
-                dist, st = sssp_with(ctx, gr, src, div)
-                print("V=%d div=%x rounds %d pull %d" % (gr.V, div, st.iterations,
-                                                          st.pull_iterations), flush=True)
+            # pull layouts: 1 window over the in-edges, 2 binned push
+            for div, layout in ((ALWAYS_PULL, 1), (ALWAYS_PULL, 2), (NEVER_PULL, 0), (32, 1),
+                                (32, 2)):
+                dist, st = sssp_with(ctx, gr, src, div, layout)
+                print("V=%d div=%x layout %d rounds %d pull %d" % (
+                    gr.V, div, layout, st.iterations, st.pull_iterations), flush=True)
                 assert (dist == ref).all()
                 if div == ALWAYS_PULL:  # a queue of out-degree-0 vertices still pushes
                     assert st.pull_iterations >= 1
                 if div == NEVER_PULL:
                     assert st.pull_iterations == 0
+
+
+def test_sssp_binned_rounds_layout_variants(ctx, oracle):
+    # binned pull rounds over layouts of several shapes: bins of 2^8 / 2^12 /
+    # 2^14 destinations, bins split over many tasks (their minima meet in a
+    # device array that must be all-infinite again after every round), a
+    # layout PageRank built first and SSSP reuses, a directed graph, and
+    # isolated vertices; every variant equals Dijkstra
+    rng = np.random.default_rng(21)
+    off, col, w = random_directed(rng, 6000, 200000, wmax=255)
+    with grus.Graph.rmat(ctx, 14, weights=(1, 255), wseed=4) as g, \
+            grus.Graph.from_csr(ctx, off, col, w, symmetric=False) as d:
+        t0 = ctx.tuning
+        try:
+            for gr, srcs in ((g, (g.info.max_degree_vertex, 3)), (d, (7, 4000))):
+                h = host_csr(gr)
+                refs = {s: oracle.sssp(h, s) for s in srcs}
+                for bits, task_edges in ((14, 0), (8, 256), (12, 1024)):
+                    t = ctx.tuning
+                    t.pr_layout = 4
+                    t.pr_bin_bits = bits
+                    t.pr_task_edges = task_edges
+                    t.sssp_pull_div = ALWAYS_PULL
+                    t.sssp_pull_layout = 2
+                    ctx.tuning = t
+                    grus.pagerank(gr, iters=1, copy=False)  # (re)builds the shared layout
+                    for s in srcs:
+                        dist, st = grus.sssp(gr, s)
+                        print("V=%d bits %d tasks %d src %d rounds %d pull %d" % (
+                            gr.V, bits, task_edges, s, st.iterations, st.pull_iterations),
+                            flush=True)
+                        assert st.pull_iterations >= 1
+                        assert (dist == refs[s]).all()
+        finally:
+            ctx.tuning = t0
 
 
 def test_sssp_grid_local_hops_invariant(ctx, oracle):
