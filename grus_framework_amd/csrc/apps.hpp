@@ -24,7 +24,11 @@ void run_sssp_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, gr
 // window pull round of sssp.hip
 struct StepCounters;
 struct BinnedMinArgs {
-  const uint64_t *front;  // frontier bitmap
+  // the frontier: vertices u with front_stamp[u] == front_id, or (when
+  // front_stamp is null) the set bits of the bitmap front
+  const uint32_t *front_stamp;
+  uint32_t front_id;
+  const uint64_t *front;
   uint32_t *dist;
   const uint64_t *out_off;
   uint32_t T_thr, near_id, far_id;

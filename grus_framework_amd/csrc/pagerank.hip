@@ -340,12 +340,21 @@ struct ContribValue {
     return __float_as_uint(contrib[u]);
   }
 };
-struct FrontDistValue {  // 0xFFFFFFFF: not in the frontier
+struct FrontDistValue {  // 0xFFFFFFFF: not in the frontier bitmap
   const uint64_t *front;
   const uint32_t *dist;
   __device__ __forceinline__ uint32_t operator()(uint32_t u) const {
     const uint32_t d = dist[u];  // both loads issued before either is used
     return bm_test(front, u) ? d : 0xFFFFFFFFu;
+  }
+};
+struct StampDistValue {  // 0xFFFFFFFF: not stamped as the current queue
+  const uint32_t *stamp;
+  uint32_t id;
+  const uint32_t *dist;
+  __device__ __forceinline__ uint32_t operator()(uint32_t u) const {
+    const uint32_t d = dist[u];
+    return stamp[u] == id ? d : 0xFFFFFFFFu;
   }
 };
 
@@ -1483,10 +1492,17 @@ void sssp_binned_round(Graph &g, const BinnedMinArgs &a) {
   Context &ctx = *g.ctx;
   PrWork &w = *g.pr;
   const int bits = w.bp_bits;
-  pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
-      w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
-      w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
-      pr::FrontDistValue{a.front, a.dist}, w.bp_val.as<uint32_t>());
+  if (a.front_stamp) {
+    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
+        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
+        w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
+        pr::StampDistValue{a.front_stamp, a.front_id, a.dist}, w.bp_val.as<uint32_t>());
+  } else {
+    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
+        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
+        w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
+        pr::FrontDistValue{a.front, a.dist}, w.bp_val.as<uint32_t>());
+  }
   GRUS_LAUNCH_CHECK();
   pr::k_bp_min<<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
       w.bp_task.as<const pr::BpTask>(), w.bp_d2.as<uint16_t>(), w.bp_w2.as<uint32_t>(),

@@ -49,11 +49,13 @@ struct SsspWork {
     cap_v = V;
     stamp = 0;
   }
+  uint64_t wraps = 0;  // stamp restarts so far
   uint32_t next_stamp(Context &ctx, uint64_t V) {
     if (stamp >= 0xFFFFFF00u) {  // wrap: clear and restart
       fill_u32(ctx, near_stamp.as<uint32_t>(), V + 1, 0);
       fill_u32(ctx, far_stamp.as<uint32_t>(), V + 1, 0);
       stamp = 0;
+      wraps++;
     }
     return ++stamp;
   }
@@ -200,12 +202,17 @@ __global__ void k_pull_min_fixup(CsrView in, const uint32_t *win_row, PullMinOp 
   window_fixup(in, win_row, k, op);
 }
 
-__global__ void k_init(uint32_t *dist, uint64_t V, uint32_t src, uint32_t *q0) {
+// the first queue is {src}, stamped like every later queue
+__global__ void k_init(uint32_t *dist, uint64_t V, uint32_t src, uint32_t *q0,
+                       uint32_t *near_stamp, uint32_t id) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < V; i += stride) {
     dist[i] = (i == src) ? 0u : INFINIT;  // SSSPInit sssp.cu:21
-    if (i == 0) q0[0] = src;
+    if (i == 0) {
+      q0[0] = src;
+      near_stamp[src] = id;
+    }
   }
 }
 
@@ -588,9 +595,19 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   const uint32_t pull_layout = ctx.tuning.sssp_pull_layout;
   const bool binned_pull = pull_edges != ~0ull && pull_layout != 1 && !persistent &&
                            g.E > 0 && sssp_binned_ready(g);
+  // a run starts far from the stamp wrap, so the stamps of the current
+  // queue identify its vertices for the whole run (binned rounds read them
+  // instead of a frontier bitmap)
+  if (w.stamp >= 0xF0000000u) {
+    fill_u32(ctx, w.near_stamp.as<uint32_t>(), V + 1, 0);
+    fill_u32(ctx, w.far_stamp.as<uint32_t>(), V + 1, 0);
+    w.stamp = 0;
+  }
+  uint32_t cur_id = w.next_stamp(ctx, V);  // stamp of the queue about to be relaxed
+  const uint64_t wraps0 = w.wraps;
   GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
-  sssp::k_init<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(dist, V, src,
-                                                                 w.q[0].as<uint32_t>());
+  sssp::k_init<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+      dist, V, src, w.q[0].as<uint32_t>(), w.near_stamp.as<uint32_t>(), cur_id);
   GRUS_LAUNCH_CHECK();
   launches++;
   if (persistent) {
@@ -612,12 +629,19 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
       const uint32_t near_id = w.next_stamp(ctx, V);
       if (binned_pull && mf > pull_edges) {
         // push over the binned layout: sequential streams instead of the
-        // window's gathers of dist[u] (one cache line per in-edge)
-        uint64_t *fbm = w.front_bm.as<uint64_t>();
-        fill_u64(ctx, fbm, W, 0);
-        queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nn, fbm, W);
+        // window's gathers of dist[u] (one cache line per in-edge).  The
+        // frontier is the vertices stamped cur_id (a bitmap only after a
+        // stamp wrap, which clears the stamps)
         BinnedMinArgs a{};
-        a.front = fbm;
+        if (w.wraps == wraps0) {
+          a.front_stamp = w.near_stamp.as<uint32_t>();
+          a.front_id = cur_id;
+        } else {
+          uint64_t *fbm = w.front_bm.as<uint64_t>();
+          fill_u64(ctx, fbm, W, 0);
+          queue_to_bitmap(ctx, w.q[cur].as<uint32_t>(), (uint32_t)nn, fbm, W);
+          a.front = fbm;
+        }
         a.dist = dist;
         a.out_off = out.off;
         a.T_thr = Tk;
@@ -685,6 +709,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
       mf = hc->next_edges;
       nfar += hc->far_count;
       cur ^= 1;
+      cur_id = near_id;
     }
     if (nfar == 0) break;
     // advance the threshold to the bucket of the smallest live far distance
@@ -714,6 +739,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     nfar = hc->far_count;
     fcur ^= 1;
     T = T_new;
+    cur_id = near_id;
   }
   GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   GRUS_HIP(hipEventSynchronize(ctx.ev1));
