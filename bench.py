@@ -138,7 +138,8 @@ def main():
                     help="alpha,beta,cutoff: use these BFS thresholds instead of autotuning "
                          "(e.g. the values an unprofiled run's autotuner printed)")
     ap.add_argument("--only", default="",
-                    help="comma list of extras to run: pagerank,cc,sssp,cu,serving,capped")
+                    help="comma list of extras to run: pagerank,cc,sssp (or sssp_rmat, "
+                         "sssp_grid),cu,serving,capped")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device-to-device copy in its own timed window")
     ap.add_argument("--regions-out", default="",
@@ -268,8 +269,9 @@ def run_extra(grus, O, ctx, g, h, threads, args):
         run_pagerank(grus, O, g, h, threads, args, out)
     if want("cc"):
         run_cc(grus, O, g, h, threads, args, out)
-    if want("sssp"):
-        run_sssp(grus, O, ctx, threads, args, out)
+    if want("sssp") or want("sssp_rmat") or want("sssp_grid"):
+        run_sssp(grus, O, ctx, threads, args, out,
+                 rmat=want("sssp") or want("sssp_rmat"), grid=want("sssp") or want("sssp_grid"))
     if want("serving"):
         out["serving_rmat_s22"] = run_serving(grus)
     if want("cu"):
@@ -328,15 +330,24 @@ def run_cc(grus, O, g, h, threads, args, out):
     log("cc: %.3f ms" % ms)
 
 
-def run_sssp(grus, O, ctx, threads, args, out):
+def run_sssp(grus, O, ctx, threads, args, out, rmat=True, grid=True):
     """SSSP on weighted RMAT s24 (w in [1,255]) and on the 4096x4096 grid."""
+    if rmat:
+        run_sssp_rmat(grus, O, ctx, threads, args, out)
+    if grid:
+        run_sssp_grid(grus, O, ctx, threads, args, out)
+
+
+def run_sssp_rmat(grus, O, ctx, threads, args, out):
     with grus.Graph.rmat(ctx, args.scale, weights=(1, 255), wseed=3) as gw:
         s = gw.info.max_degree_vertex
         grus.sssp(gw, s, copy=False)  # first call builds the binned layout
         ms = []
+        REGIONS.begin("sssp_rmat_s%d_traversal" % args.scale)
         for _ in range(3):
             dist, st = grus.sssp(gw, s, copy=True)
             ms.append(st.device_ms)
+        REGIONS.end(3)
         ms = float(np.median(ms))
         rm = {"ms": round(ms, 2), "GTEPS": round(st.edges_reached / ms / 1e6, 3),
               "rounds": st.iterations, "pull_rounds": st.pull_iterations,
@@ -353,7 +364,9 @@ def run_sssp(grus, O, ctx, threads, args, out):
         log("sssp rmat: %.2f ms%s" % (ms, (", cpu %.1f ms (equal: %s)" % (
             rm["cpu_ms"], rm["distances_equal_cpu"])) if "cpu_ms" in rm else ""))
 
-    # ---- SSSP on the 4096x4096 weighted grid (source 0) ----
+
+def run_sssp_grid(grus, O, ctx, threads, args, out):
+    """SSSP on the 4096x4096 grid, weights in [1,1000], source 0."""
     with grus.Graph.grid(ctx, 4096, 4096, weights=(1, 1000), wseed=2) as gg:
         grus.sssp(gg, 0, copy=False)
         _, st = grus.sssp(gg, 0, copy=False)
