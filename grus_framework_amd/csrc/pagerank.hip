@@ -98,6 +98,7 @@ struct PrWork {
   // stream order (slab-transposed like d2); gmin = minima of the bins split
   // over several tasks (all 0xFFFFFFFF between rounds)
   Buf bp_w2, bp_gmin;
+  int bp_w2_bytes = 4;  // 1, 2 or 4: the narrowest that holds the largest weight
   const void *bp_w2_for = nullptr;
   const void *bp_unusable_for = nullptr;  // offsets a failed build was tried on
 };
@@ -690,8 +691,32 @@ __device__ __forceinline__ void finish_rows(uint64_t v0, uint32_t nb, Best best_
 // w2 is stored like d2, so lane l's 32 bytes of a slab are the weights of
 // its entries l, l + 64, ...  A bin's only task finishes its rows; pieces of
 // split bins take the minimum into gmin, which k_bp_min_finish completes.
+// lane l's 8 weights of slab s (entries l, l + 64, ...): 8, 16 or 32 bytes
+template <typename WT>
+struct W2Slab {
+  uint32_t v[BP_EPT];
+  __device__ __forceinline__ void load(const WT *w2, uint64_t o) {
+    if constexpr (sizeof(WT) == 1) {
+      const uint64_t x = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(w2 + o));
+#pragma unroll
+      for (int q = 0; q < BP_EPT; q++) v[q] = (uint32_t)(x >> (8 * q)) & 0xFFu;
+    } else if constexpr (sizeof(WT) == 2) {
+      const u32x4 x = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o));
+      const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int q = 0; q < BP_EPT; q++) v[q] = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
+    } else {
+      const u32x4 x0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o));
+      const u32x4 x1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o) + 1);
+      v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
+      v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+    }
+  }
+};
+
+template <typename WT>
 __global__ __launch_bounds__(BP_BLOCK) void k_bp_min(const BpTask *tasks, const uint16_t *d2,
-                                                     const uint32_t *w2, const uint32_t *gs512,
+                                                     const WT *w2, const uint32_t *gs512,
                                                      const uint32_t *val, uint64_t V, int bits,
                                                      BinnedMinArgs a, uint32_t *gmin) {
   __shared__ uint32_t acc[1 << BP_MAX_BITS];
@@ -706,20 +731,21 @@ __global__ __launch_bounds__(BP_BLOCK) void k_bp_min(const BpTask *tasks, const 
   uint32_t *wv = win[wid];
   const uint32_t s1 = (uint32_t)((t.e1 + (uint64_t)BP_SLAB - 1) / BP_SLAB);
   uint32_t s = t.e0 / BP_SLAB + wid;
-  u32x4 nx = {0, 0, 0, 0}, nw0 = {0, 0, 0, 0}, nw1 = {0, 0, 0, 0};
+  u32x4 nx = {0, 0, 0, 0};
+  W2Slab<WT> nw{};
   uint32_t ngs = 0;
   if (s < s1) {
     const uint64_t o = (uint64_t)s * BP_SLAB + lane * BP_EPT;
     nx = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(d2 + o));
-    nw0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o));
-    nw1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o) + 1);
+    nw.load(w2, o);
     ngs = gs512[s];
   }
   for (; s < s1; s += NW) {
-    const u32x4 cur = nx, cw0 = nw0, cw1 = nw1;
+    const u32x4 cur = nx;
+    const W2Slab<WT> cw = nw;
     const uint32_t gbase = ngs;
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
-    const uint32_t wt[BP_EPT] = {cw0.x, cw0.y, cw0.z, cw0.w, cw1.x, cw1.y, cw1.z, cw1.w};
+    const uint32_t *wt = cw.v;
     uint32_t ent[BP_EPT];
 #pragma unroll
     for (int q = 0; q < BP_EPT; q++) ent[q] = (q & 1) ? (wd[q >> 1] >> 16) : (wd[q >> 1] & 0xFFFFu);
@@ -741,8 +767,7 @@ __global__ __launch_bounds__(BP_BLOCK) void k_bp_min(const BpTask *tasks, const 
       const uint32_t sn = s + NW < s1 ? s + NW : s1 - 1u;
       const uint64_t o = (uint64_t)sn * BP_SLAB + lane * BP_EPT;
       nx = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(d2 + o));
-      nw0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o));
-      nw1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(w2 + o) + 1);
+      nw.load(w2, o);
       ngs = gs512[sn];
     }
 #pragma unroll
@@ -807,16 +832,30 @@ __global__ void k_bp_w2_keys(const uint64_t *off, const uint32_t *col, const uin
     }
 }
 
-// w2: the sorted weights, padded with zeros to a multiple of BP_SLAB, each
-// slab stored transposed as d2
-__global__ void k_bp_w2(const uint32_t *wt, uint64_t E, uint64_t Epad, uint32_t *w2) {
+// w2: the sorted weights (narrowed to WT, which holds the largest), padded
+// with zeros to a multiple of BP_SLAB, each slab stored transposed as d2
+template <typename WT>
+__global__ void k_bp_w2(const uint32_t *wt, uint64_t E, uint64_t Epad, WT *w2) {
   uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; p < Epad; p += stride) {
     const uint64_t r = p % BP_SLAB;
     const uint64_t phys = p - r + (r % WAVE) * BP_EPT + r / WAVE;
-    w2[phys] = p < E ? wt[p] : 0u;
+    w2[phys] = p < E ? (WT)wt[p] : (WT)0;
   }
+}
+
+__global__ void k_max_u32(const uint32_t *a, uint64_t n, uint32_t *out) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t m = 0;
+  for (; i < n; i += stride) m = a[i] > m ? a[i] : m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t y = __shfl_xor(m, o);
+    m = y > m ? y : m;
+  }
+  if (lane_id() == 0 && m) atomicMax(out, m);
 }
 
 // ---- binned push layout construction ----
@@ -1445,12 +1484,28 @@ bool build_binned_weights(Graph &g, PrWork &w) {
     pr::k_bp_w2_keys<<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
         out.off, out.col, out.w, V, w.bp_bits, ka.as<uint64_t>(), va.as<uint32_t>());
     GRUS_LAUNCH_CHECK();
+    // the stream is as narrow as the largest weight allows (RMAT inputs
+    // with weights below 256 read one byte per edge instead of four)
+    uint32_t *wmax = kb.as<uint32_t>();
+    GRUS_HIP(hipMemsetAsync(wmax, 0, 4, ctx.stream));
+    pr::k_max_u32<<<grid_cap(ctx, E, BLOCK, 4), BLOCK, 0, ctx.stream>>>(va.as<uint32_t>(), E,
+                                                                       wmax);
+    GRUS_LAUNCH_CHECK();
+    uint32_t hmax = 0;
+    GRUS_HIP(hipMemcpyAsync(&hmax, wmax, 4, hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    w.bp_w2_bytes = hmax < 256u ? 1 : hmax < 65536u ? 2 : 4;
     uint32_t *wt = nullptr;
     sort_pairs_u64_u32(ctx, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<uint32_t>(),
                        vb.as<uint32_t>(), E, &wt, std::max(1, bits_for(w.bp_nbins - 1)));
-    w.bp_w2.alloc(Epad * 4);
-    pr::k_bp_w2<<<grid_cap(ctx, Epad, BLOCK), BLOCK, 0, ctx.stream>>>(wt, E, Epad,
-                                                                     w.bp_w2.as<uint32_t>());
+    w.bp_w2.alloc(Epad * (uint64_t)w.bp_w2_bytes);
+    const unsigned grid = grid_cap(ctx, Epad, BLOCK);
+    if (w.bp_w2_bytes == 1)
+      pr::k_bp_w2<<<grid, BLOCK, 0, ctx.stream>>>(wt, E, Epad, w.bp_w2.as<uint8_t>());
+    else if (w.bp_w2_bytes == 2)
+      pr::k_bp_w2<<<grid, BLOCK, 0, ctx.stream>>>(wt, E, Epad, w.bp_w2.as<uint16_t>());
+    else
+      pr::k_bp_w2<<<grid, BLOCK, 0, ctx.stream>>>(wt, E, Epad, w.bp_w2.as<uint32_t>());
     GRUS_LAUNCH_CHECK();
     ctx.sync();  // the sort buffers are freed on scope exit
   }
@@ -1504,10 +1559,20 @@ void sssp_binned_round(Graph &g, const BinnedMinArgs &a) {
         pr::FrontDistValue{a.front, a.dist}, w.bp_val.as<uint32_t>());
   }
   GRUS_LAUNCH_CHECK();
-  pr::k_bp_min<<<(unsigned)w.bp_ntask, pr::BP_BLOCK, 0, ctx.stream>>>(
-      w.bp_task.as<const pr::BpTask>(), w.bp_d2.as<uint16_t>(), w.bp_w2.as<uint32_t>(),
-      w.bp_gs512.as<uint32_t>(), w.bp_val.as<uint32_t>(), g.V, bits, a,
-      w.bp_gmin.as<uint32_t>());
+  const pr::BpTask *tasks = w.bp_task.as<const pr::BpTask>();
+  const uint16_t *d2 = w.bp_d2.as<uint16_t>();
+  const uint32_t *gs = w.bp_gs512.as<uint32_t>(), *val = w.bp_val.as<uint32_t>();
+  uint32_t *gmin = w.bp_gmin.as<uint32_t>();
+  const unsigned nt = (unsigned)w.bp_ntask;
+  if (w.bp_w2_bytes == 1)
+    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint8_t>(), gs, val,
+                                                      g.V, bits, a, gmin);
+  else if (w.bp_w2_bytes == 2)
+    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint16_t>(), gs, val,
+                                                      g.V, bits, a, gmin);
+  else
+    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint32_t>(), gs, val,
+                                                      g.V, bits, a, gmin);
   GRUS_LAUNCH_CHECK();
   if (w.bp_nmulti) {
     pr::k_bp_min_finish<<<(unsigned)w.bp_nmulti, pr::BP_BLOCK, 0, ctx.stream>>>(

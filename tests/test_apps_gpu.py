@@ -440,13 +440,18 @@ def test_sssp_binned_rounds_layout_variants(ctx, oracle):
     # device array that must be all-infinite again after every round), a
     # layout PageRank built first and SSSP reuses, a directed graph, and
     # isolated vertices; every variant equals Dijkstra
+    # weight streams of 1, 2 and 4 bytes: largest weights 255, 1000, 300000
     rng = np.random.default_rng(21)
     off, col, w = random_directed(rng, 6000, 200000, wmax=255)
+    off4, col4, w4 = random_directed(rng, 5000, 150000, wmax=300000)
     with grus.Graph.rmat(ctx, 14, weights=(1, 255), wseed=4) as g, \
-            grus.Graph.from_csr(ctx, off, col, w, symmetric=False) as d:
+            grus.Graph.rmat(ctx, 13, weights=(1, 1000), wseed=5) as g2, \
+            grus.Graph.from_csr(ctx, off, col, w, symmetric=False) as d, \
+            grus.Graph.from_csr(ctx, off4, col4, w4, symmetric=False) as d4:
         t0 = ctx.tuning
         try:
-            for gr, srcs in ((g, (g.info.max_degree_vertex, 3)), (d, (7, 4000))):
+            for gr, srcs in ((g, (g.info.max_degree_vertex, 3)), (g2, (g2.info.max_degree_vertex,)),
+                             (d, (7, 4000)), (d4, (1, 2500))):
                 h = host_csr(gr)
                 refs = {s: oracle.sssp(h, s) for s in srcs}
                 for bits, task_edges in ((14, 0), (8, 256), (12, 1024)):
