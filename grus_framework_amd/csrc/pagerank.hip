@@ -688,10 +688,12 @@ __device__ __forceinline__ void finish_rows(uint64_t v0, uint32_t nb, Best best_
 // frontier sources (0xFFFFFFFF for the others); entry e of a bin's stream
 // offers val[group(e)] + w2[e] to its destination, and the minimum is kept
 // in LDS (exact and order-independent).  The stream walk is k_bp_accum's;
-// w2 is stored like d2, so lane l's 32 bytes of a slab are the weights of
-// its entries l, l + 64, ...  A bin's only task finishes its rows; pieces of
-// split bins take the minimum into gmin, which k_bp_min_finish completes.
-// lane l's 8 weights of slab s (entries l, l + 64, ...): 8, 16 or 32 bytes
+// w2 is stored like d2, so lane l's 8, 16 or 32 bytes of a slab (weights of
+// 1, 2 or 4 bytes) are the weights of its entries l, l + 64, ...  A bin's
+// only task finishes its rows; pieces of split bins take the minimum into
+// gmin, which k_bp_min_finish completes.
+
+// lane l's 8 weights of a slab (entries l, l + 64, ...)
 template <typename WT>
 struct W2Slab {
   uint32_t v[BP_EPT];
