@@ -295,7 +295,7 @@ class Server:
     def __init__(self, g, max_inflight_batches=0, max_batch=0, state_budget_bytes=0,
                  copy_results=False, admit_min_free=0):
         """Zero for max_inflight_batches, max_batch or admit_min_free picks the
-        library default (2 engines per kind, 64 lanes, admit into idle engines)."""
+        library default (4 engines per kind, 64 lanes, admit into idle engines)."""
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
                                int(copy_results), admit_min_free)

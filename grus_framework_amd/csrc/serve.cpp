@@ -11,14 +11,17 @@
 // result is digested (and copied to the caller's buffer when requested) and
 // the slot is freed for the next query.
 //
-// The default admits only into an idle engine, two engines per kind.  Lanes
+// The default admits only into an idle engine, four engines per kind.  Lanes
 // of one batch then stay in step: on small-world graphs their frontiers cover
 // mostly the same vertices at the same step, and a step of the engine costs
 // about one sweep of their union.  Lanes admitted one at a time drift apart,
 // every step sweeps a large union, and the engine spends its bandwidth on few
-// queries at a time (RMAT scale 22, 50/50 BFS/SSSP, saturated, 2 engines per
-// kind: 2470 queries/s for idle-only admission against 1300 for any-free-lane
-// admission; p99 latency at 60% load 0.18 s against 0.74 s).
+// queries at a time (RMAT scale 22, 50/50 BFS/SSSP, saturated, 4 engines per
+// kind: 2780 queries/s for idle-only admission against 1760 for any-free-lane
+// admission).  Four engines overlap the small early and late steps of their
+// batches: 2780 queries/s against 2470 with two, at a higher p99 under
+// moderate load (0.4-0.6 s against 0.2 s), when the engines' small batches
+// share the device.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -50,7 +53,7 @@ struct Query {
 class ServerImpl : public Server {
  public:
   ServerImpl(Graph &g, const grus_server_opts *o) : g_(g) {
-    opts_.max_inflight_batches = 2;
+    opts_.max_inflight_batches = 4;
     opts_.max_batch = 64;
     opts_.state_budget_bytes = 0;
     opts_.copy_results = 0;

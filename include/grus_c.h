@@ -258,9 +258,10 @@ int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
 typedef struct grus_server_opts {
   int max_inflight_batches;   /* concurrent batches per query kind (engines,
                                  each with its own stream and worker thread),
-                                 1..8, default 2: a second engine overlaps one
-                                 engine's per-step host round trips; beyond
-                                 that, engines mostly share the device */
+                                 1..8, default 4: engines overlap each other's
+                                 small steps and host round trips (more
+                                 throughput at saturation; under moderate load
+                                 fewer engines give lower tail latency) */
   int max_batch;              /* lanes per batch, 1..64, default 64 */
   uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
   int copy_results;           /* 1: copy each result to the host buffer given
