@@ -474,6 +474,21 @@ def test_sssp_binned_rounds_layout_variants(ctx, oracle):
             ctx.tuning = t0
 
 
+def test_tuning_rejects_out_of_range_layouts(ctx):
+    t0 = ctx.tuning
+    try:
+        for field, bad in (("sssp_pull_layout", 3), ("pr_layout", 5), ("pr_bin_bits", 5),
+                           ("pr_task_edges", 100)):
+            t = ctx.tuning
+            setattr(t, field, bad)
+            with pytest.raises(grus.GrusError) as e:
+                ctx.tuning = t
+            assert e.value.code == capi.E_INVALID, field
+        assert ctx.tuning.sssp_pull_layout == t0.sssp_pull_layout  # left unchanged
+    finally:
+        ctx.tuning = t0
+
+
 def test_sssp_grid_local_hops_invariant(ctx, oracle):
     # the persistent kernel's in-workgroup hop passes only add relaxations:
     # distances equal Dijkstra's for any pass count
