@@ -145,6 +145,7 @@ class ServerImpl : public Server {
     MsEngine &eng = *engines_[ei];
     const int kind = (int)eng.kind();
     std::vector<Query> lane_q(64);
+    uint64_t done_lanes = 0;  // finished lanes not yet digested and released
     try {
       GRUS_HIP(hipSetDevice(g_.ctx->device));
       for (;;) {
@@ -174,18 +175,25 @@ class ServerImpl : public Server {
           }
         }
         if (!eng.active()) continue;
-        const std::vector<int> fin = eng.step();
-        if (fin.empty()) continue;
-        // one digest pass for every lane that finished in this step
-        uint64_t mask = 0, reached[64], checksum[64];
-        for (int lane : fin) mask |= 1ull << lane;
-        eng.digest(mask, reached, checksum);
-        for (int lane : fin) {
+        for (int lane : eng.step()) done_lanes |= 1ull << lane;
+        // Finished lanes are digested together (one pass over the engine's
+        // state, which for SSSP is the whole [V][64] array) once no lane is
+        // running or once they are enough to admit new queries into: under
+        // whole-batch admission that is one pass per batch instead of one
+        // per step in which a lane finished, and no result waits longer
+        // than its lanes could be reused.
+        const int ndone = __builtin_popcountll(done_lanes);
+        if (!done_lanes || (eng.busy_mask() != done_lanes && ndone < min_free_)) continue;
+        uint64_t reached[64], checksum[64];
+        eng.digest(done_lanes, reached, checksum);
+        for (uint64_t m = done_lanes; m; m &= m - 1) {
+          const int lane = __builtin_ctzll(m);
           const Query &q = lane_q[lane];
           if (opts_.copy_results) eng.extract(lane, q.out, nullptr, nullptr);
           eng.release(lane);
           finish(q, GRUS_OK, reached[lane], checksum[lane]);
         }
+        done_lanes = 0;
       }
     } catch (const Error &e) {
       fail(e.code, e.what());
