@@ -231,13 +231,15 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t x, uint32_t i) {
 // read of dist[v] that already shows nd below it proves that dist[v] drops
 // this round: the atomics are issued without waiting for their old values, and
 // several edges' row reads are in flight before the first atomic.
+// The chunk count is read from the scan (CS[F]); the grid, sized by the
+// host from an estimate, only sets how many waves share the chunks.
 __global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t *q, uint32_t F,
-                                                      const uint64_t *CS, uint64_t chunks,
-                                                      const uint64_t *front, uint64_t *next,
-                                                      uint32_t *dist) {
+                                                      const uint64_t *CS, const uint64_t *front,
+                                                      uint64_t *next, uint32_t *dist) {
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
   const uint32_t lane = lane_id();
+  const uint64_t chunks = CS[F];
   // each wave takes a contiguous run of chunks (equal counts, so equal work
   // within one chunk).  The queue entries its chunks belong to are read 64 at
   // a time, one per lane (scan value, vertex, row bounds, front word): the
@@ -651,13 +653,14 @@ std::vector<int> MsEngine::step() {
           out.off, queue_.as<uint32_t>(), (uint32_t)qn_, push_.deg.as<uint64_t>());
       GRUS_LAUNCH_CHECK();
       scan_excl_u64(ctx, push_.deg.as<uint64_t>(), cs.as<uint64_t>(), qn_ + 1);
-      const uint64_t chunks = read_u64(cs.as<uint64_t>() + qn_);
-      if (chunks) {
-        ms::k_sssp_relax<<<grid_cap(ctx, chunks * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-            out, queue_.as<uint32_t>(), (uint32_t)qn_, cs.as<uint64_t>(), chunks,
-            front_.as<uint64_t>(), next_.as<uint64_t>(), vals_.as<uint32_t>());
-        GRUS_LAUNCH_CHECK();
-      }
+      // no read-back of the chunk count: at most one chunk per queued vertex
+      // plus one per CHUNK of its edges (front_edges_ misses the degrees of
+      // sources admitted since the last step, which only sizes the grid)
+      const uint64_t est = qn_ + front_edges_ / ms::CHUNK;
+      ms::k_sssp_relax<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+          out, queue_.as<uint32_t>(), (uint32_t)qn_, cs.as<uint64_t>(), front_.as<uint64_t>(),
+          next_.as<uint64_t>(), vals_.as<uint32_t>());
+      GRUS_LAUNCH_CHECK();
     }
     ms::k_sssp_finalize<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
         out.off, front_.as<uint64_t>(), next_.as<uint64_t>(), V, running, queue_.as<uint32_t>(),
