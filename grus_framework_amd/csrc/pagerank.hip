@@ -359,11 +359,14 @@ struct StampDistValue {  // 0xFFFFFFFF: not stamped as the current queue
   }
 };
 
-template <class Src>
+//
+// The values are stored as VT: u32, or u16 for SSSP rounds whose frontier
+// distances are all below 0xFFFF (0xFFFF then marks "not in the frontier").
+template <class Src, typename VT = uint32_t>
 __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, const uint32_t *ch_i0,
                                                       const uint32_t *ch_r0, const uint16_t *p1,
                                                       const uint32_t *rbase, const uint32_t *rtab,
-                                                      uint32_t nbins, Src src, uint32_t *val) {
+                                                      uint32_t nbins, Src src, VT *val) {
   extern __shared__ uint32_t sr[];  // the chunk's run offsets (<= nbins), then BP_SPAN values
   uint32_t *sc = sr + nbins;
   const uint64_t c = blockIdx.x;
@@ -435,13 +438,13 @@ __global__ __launch_bounds__(BLOCK) void k_bp_scatter(const uint32_t *ch_u0, con
 #pragma unroll
       for (int q = 0; q < BP_EPT; q++) {
         const uint32_t i = sb + WAVE * q + lane;
-        val[sr[rq[q] - r0] + i] = sc[ent[q] & 0x1FFFu];
+        val[sr[rq[q] - r0] + i] = (VT)sc[ent[q] & 0x1FFFu];
       }
     } else {
 #pragma unroll
       for (int q = 0; q < BP_EPT; q++) {
         const uint32_t i = sb + WAVE * q + lane;
-        if (i >= i0 && i < i1) val[sr[rq[q] - r0] + i] = sc[ent[q] & 0x1FFFu];
+        if (i >= i0 && i < i1) val[sr[rq[q] - r0] + i] = (VT)sc[ent[q] & 0x1FFFu];
       }
     }
   }
@@ -716,10 +719,10 @@ struct W2Slab {
   }
 };
 
-template <typename WT>
+template <typename WT, typename VT>
 __global__ __launch_bounds__(BP_BLOCK) void k_bp_min(const BpTask *tasks, const uint16_t *d2,
                                                      const WT *w2, const uint32_t *gs512,
-                                                     const uint32_t *val, uint64_t V, int bits,
+                                                     const VT *val, uint64_t V, int bits,
                                                      BinnedMinArgs a, uint32_t *gmin) {
   __shared__ uint32_t acc[1 << BP_MAX_BITS];
   __shared__ uint32_t win[BP_BLOCK / WAVE][BP_SLAB];
@@ -764,7 +767,10 @@ __global__ __launch_bounds__(BP_BLOCK) void k_bp_min(const BpTask *tasks, const 
     const uint32_t w0 = (b0 & 1ull) ? gbase : gbase - 1u;
     uint32_t wl[BP_SLAB / WAVE];
 #pragma unroll
-    for (int j = 0; j < BP_SLAB / WAVE; j++) wl[j] = val[w0 + lane + WAVE * j];
+    for (int j = 0; j < BP_SLAB / WAVE; j++) {
+      const uint32_t x = val[w0 + lane + WAVE * j];
+      wl[j] = (sizeof(VT) == 2 && x == 0xFFFFu) ? 0xFFFFFFFFu : x;  // u16: 0xFFFF = none
+    }
     {
       const uint32_t sn = s + NW < s1 ? s + NW : s1 - 1u;
       const uint64_t o = (uint64_t)sn * BP_SLAB + lane * BP_EPT;
@@ -1545,41 +1551,54 @@ bool sssp_binned_ready(Graph &g) {
   return true;
 }
 
-void sssp_binned_round(Graph &g, const BinnedMinArgs &a) {
+namespace {
+template <class Src, typename VT>
+void binned_min_launch(Graph &g, PrWork &w, const BinnedMinArgs &a, Src src) {
   Context &ctx = *g.ctx;
-  PrWork &w = *g.pr;
   const int bits = w.bp_bits;
-  if (a.front_stamp) {
-    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
-        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
-        w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
-        pr::StampDistValue{a.front_stamp, a.front_id, a.dist}, w.bp_val.as<uint32_t>());
-  } else {
-    pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
-        w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
-        w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
-        pr::FrontDistValue{a.front, a.dist}, w.bp_val.as<uint32_t>());
-  }
+  VT *val = w.bp_val.as<VT>();
+  pr::k_bp_scatter<<<(unsigned)w.bp_nch, BLOCK, (w.bp_nbins + pr::BP_SPAN) * 4, ctx.stream>>>(
+      w.bp_ch_u0.as<uint32_t>(), w.bp_ch_i0.as<uint32_t>(), w.bp_ch_r0.as<uint32_t>(),
+      w.bp_p1.as<uint16_t>(), w.bp_rbase.as<uint32_t>(), w.bp_rtab.as<uint32_t>(), w.bp_nbins,
+      src, val);
   GRUS_LAUNCH_CHECK();
   const pr::BpTask *tasks = w.bp_task.as<const pr::BpTask>();
   const uint16_t *d2 = w.bp_d2.as<uint16_t>();
-  const uint32_t *gs = w.bp_gs512.as<uint32_t>(), *val = w.bp_val.as<uint32_t>();
+  const uint32_t *gs = w.bp_gs512.as<uint32_t>();
   uint32_t *gmin = w.bp_gmin.as<uint32_t>();
   const unsigned nt = (unsigned)w.bp_ntask;
   if (w.bp_w2_bytes == 1)
-    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint8_t>(), gs, val,
-                                                      g.V, bits, a, gmin);
+    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint8_t>(), gs,
+                                                      (const VT *)val, g.V, bits, a, gmin);
   else if (w.bp_w2_bytes == 2)
-    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint16_t>(), gs, val,
-                                                      g.V, bits, a, gmin);
+    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint16_t>(), gs,
+                                                      (const VT *)val, g.V, bits, a, gmin);
   else
-    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint32_t>(), gs, val,
-                                                      g.V, bits, a, gmin);
+    pr::k_bp_min<<<nt, pr::BP_BLOCK, 0, ctx.stream>>>(tasks, d2, w.bp_w2.as<uint32_t>(), gs,
+                                                      (const VT *)val, g.V, bits, a, gmin);
   GRUS_LAUNCH_CHECK();
   if (w.bp_nmulti) {
     pr::k_bp_min_finish<<<(unsigned)w.bp_nmulti, pr::BP_BLOCK, 0, ctx.stream>>>(
-        w.bp_multi.as<uint32_t>(), bits, g.V, w.bp_gmin.as<uint32_t>(), a);
+        w.bp_multi.as<uint32_t>(), bits, g.V, gmin, a);
     GRUS_LAUNCH_CHECK();
+  }
+}
+}  // namespace
+
+void sssp_binned_round(Graph &g, const BinnedMinArgs &a) {
+  PrWork &w = *g.pr;
+  // the frontier's distances are below the round's near threshold, so a
+  // threshold up to 0xFFFF lets the group values travel as u16 (half the
+  // value stream written in phase 1 and read in phase 2)
+  const bool narrow = a.T_thr <= 0xFFFFu;
+  if (a.front_stamp) {
+    const pr::StampDistValue src{a.front_stamp, a.front_id, a.dist};
+    if (narrow) binned_min_launch<pr::StampDistValue, uint16_t>(g, w, a, src);
+    else binned_min_launch<pr::StampDistValue, uint32_t>(g, w, a, src);
+  } else {
+    const pr::FrontDistValue src{a.front, a.dist};
+    if (narrow) binned_min_launch<pr::FrontDistValue, uint16_t>(g, w, a, src);
+    else binned_min_launch<pr::FrontDistValue, uint32_t>(g, w, a, src);
   }
 }
 
