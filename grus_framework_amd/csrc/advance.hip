@@ -89,8 +89,12 @@ __global__ void k_fill_u64(uint64_t *a, uint64_t n, uint64_t v) {
   for (; i < n; i += stride) a[i] = v;
 }
 
-__global__ void k_reached(const uint64_t *off, const uint32_t *val, uint64_t V,
-                          unsigned long long *acc) {
+// reached vertices and their out-edges; one atomic pair per workgroup (a
+// pair from every wave onto the same two addresses serialised at the L2:
+// 0.4 ms at V = 2^24)
+__global__ __launch_bounds__(BLOCK) void k_reached(const uint64_t *off, const uint32_t *val,
+                                                   uint64_t V, unsigned long long *acc) {
+  __shared__ unsigned long long s_n[BLOCK / WAVE], s_e[BLOCK / WAVE];
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long n = 0, e = 0;
@@ -102,9 +106,19 @@ __global__ void k_reached(const uint64_t *off, const uint32_t *val, uint64_t V,
   }
   n = wave_sum(n);
   e = wave_sum(e);
+  const uint32_t wid = threadIdx.x / WAVE;
   if (lane_id() == 0) {
-    atomicAdd(&acc[0], n);
-    atomicAdd(&acc[1], e);
+    s_n[wid] = n;
+    s_e[wid] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < BLOCK / WAVE; w++) {
+      n += s_n[w];
+      e += s_e[w];
+    }
+    if (n) atomicAdd(&acc[0], n);
+    if (e) atomicAdd(&acc[1], e);
   }
 }
 
@@ -115,7 +129,7 @@ void reached_stats(Context &ctx, const uint64_t *off, const uint32_t *val, uint6
   auto *acc = reinterpret_cast<unsigned long long *>(ctx.d_ctl.p) + 40;
   GRUS_HIP(hipMemsetAsync(acc, 0, 16, ctx.stream));
   if (V) {
-    k_reached<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(off, val, V, acc);
+    k_reached<<<grid_cap(ctx, V, BLOCK, 4), BLOCK, 0, ctx.stream>>>(off, val, V, acc);
     GRUS_LAUNCH_CHECK();
   }
   ctx.read_ctl(42);

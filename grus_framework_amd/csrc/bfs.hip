@@ -147,12 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_stamp_commit(const uint8_t *stamp, ui
       }
     }
   }
-  found = wave_sum(found);
-  edges = wave_sum(edges);
-  if (lane_id() == 0 && found) {
-    atomicAdd(&cnt->next_count, found);
-    atomicAdd((unsigned long long *)&cnt->next_edges, edges);
-  }
+  block_add_counts(found, edges, &cnt->next_count, (unsigned long long *)&cnt->next_edges);
 }
 
 __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uint64_t *unreachable,
@@ -263,12 +258,8 @@ __global__ __launch_bounds__(BLOCK) void k_pull(CsrView in, const uint64_t *out_
       if (is_long) longq[slot] = v;
     }
   }
-  my_found = wave_sum(my_found);
-  my_edges = wave_sum(my_edges);
-  if (lane == 0 && my_found) {
-    atomicAdd(&cnt->next_count, my_found);
-    atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
-  }
+  block_add_counts(my_found, my_edges, &cnt->next_count,
+                   (unsigned long long *)&cnt->next_edges);
 }
 
 // Bottom-up step over a candidate list instead of the whole bitmap, for
@@ -348,12 +339,8 @@ __global__ __launch_bounds__(BLOCK) void k_pull_sparse(
       if (is_long) longq[slot] = v[j];
     }
   }
-  my_found = wave_sum(my_found);
-  my_edges = wave_sum(my_edges);
-  if (lane == 0 && my_found) {
-    atomicAdd(&cnt->next_count, my_found);
-    atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
-  }
+  block_add_counts(my_found, my_edges, &cnt->next_count,
+                   (unsigned long long *)&cnt->next_edges);
 }
 
 // wave-cooperative scan of the vertices whose first K in-edges held no parent

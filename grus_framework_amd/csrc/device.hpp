@@ -71,6 +71,36 @@ __device__ __forceinline__ T wave_sum(T x) {
   return x;
 }
 
+// Adds every thread's (count, sum) to two device counters with one atomic
+// pair per workgroup (skipped when the count is zero).  Every thread of the
+// workgroup must call it, at the end of the kernel; blockDim.x <= 1024.
+// An atomic pair from every wave lands on the same two addresses and
+// serialises at the L2.
+__device__ __forceinline__ void block_add_counts(uint32_t count, unsigned long long sum,
+                                                 uint32_t *count_out,
+                                                 unsigned long long *sum_out) {
+  __shared__ uint32_t s_c[1024 / WAVE];
+  __shared__ unsigned long long s_s[1024 / WAVE];
+  count = wave_sum(count);
+  sum = wave_sum(sum);
+  const uint32_t wid = threadIdx.x / WAVE, nwaves = blockDim.x / WAVE;
+  if (lane_id() == 0) {
+    s_c[wid] = count;
+    s_s[wid] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < nwaves; w++) {
+      count += s_c[w];
+      sum += s_s[w];
+    }
+    if (count) {
+      atomicAdd(count_out, count);
+      atomicAdd(sum_out, sum);
+    }
+  }
+}
+
 __device__ __forceinline__ bool bm_test(const uint64_t *bm, uint32_t v) {
   return (bm[v >> 6] >> (v & 63)) & 1ull;
 }
