@@ -151,8 +151,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
     }
     __syncthreads();
   }
-  my_edges = wave_sum(my_edges);
-  if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+  block_add_u64(my_edges, (unsigned long long *)&cnt->next_edges);
 }
 
 // The same push for a small frontier (F <= PUSH_SMALL_F) in one launch:
@@ -239,8 +238,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_small(CsrView g, const uint32_t 
       if (gen == 2) qfar[fslot] = dst;
     }
   }
-  my_edges = wave_sum(my_edges);
-  if (lane == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+  block_add_u64(my_edges, (unsigned long long *)&cnt->next_edges);
 }
 
 // Host side of the push operator: degree scan + edge-balanced expansion

@@ -101,6 +101,19 @@ __device__ __forceinline__ void block_add_counts(uint32_t count, unsigned long l
   }
 }
 
+// The same for one 64-bit sum (skipped when zero).
+__device__ __forceinline__ void block_add_u64(unsigned long long sum, unsigned long long *out) {
+  __shared__ unsigned long long s_s[1024 / WAVE];
+  sum = wave_sum(sum);
+  const uint32_t wid = threadIdx.x / WAVE, nwaves = blockDim.x / WAVE;
+  if (lane_id() == 0) s_s[wid] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < nwaves; w++) sum += s_s[w];
+    if (sum) atomicAdd(out, sum);
+  }
+}
+
 __device__ __forceinline__ bool bm_test(const uint64_t *bm, uint32_t v) {
   return (bm[v >> 6] >> (v & 63)) & 1ull;
 }

@@ -200,8 +200,7 @@ __global__ __launch_bounds__(BLOCK) void k_bfs_finalize(const uint64_t *off, uin
   }
   __syncthreads();
   if (threadIdx.x == 0 && s_alive) atomicOr((unsigned long long *)&cnt->alive, s_alive);
-  edges = wave_sum(edges);
-  if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
+  block_add_u64(edges, (unsigned long long *)&cnt->front_edges);
 }
 
 // ---- SSSP ----
@@ -351,8 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, ui
   }
   __syncthreads();
   if (threadIdx.x == 0 && s_alive) atomicOr((unsigned long long *)&cnt->alive, s_alive);
-  edges = wave_sum(edges);
-  if (lane_id() == 0 && edges) atomicAdd((unsigned long long *)&cnt->front_edges, edges);
+  block_add_u64(edges, (unsigned long long *)&cnt->front_edges);
 }
 
 // the step's counters to the pinned host mirror, cleared for the next step

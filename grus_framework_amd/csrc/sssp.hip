@@ -263,8 +263,7 @@ __global__ void k_far_split(const uint32_t *far, uint32_t n, const uint32_t *dis
     uint32_t s2 = wave_append(&cnt->far_count, to == 2);
     if (to == 2) far_out[s2] = v;
   }
-  my_edges = wave_sum(my_edges);
-  if (lane_id() == 0 && my_edges) atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
+  block_add_u64(my_edges, (unsigned long long *)&cnt->next_edges);
 }
 
 __global__ void k_weight_sum(const uint32_t *w, uint64_t E, unsigned long long *out) {
