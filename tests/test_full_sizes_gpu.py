@@ -1,0 +1,138 @@
+"""GPU parity at the full sizes of the benchmark workloads.
+
+- RMAT scale 24 (edge factor 16, a/b/c = .57/.19/.19, seed 1, symmetrized,
+  16,777,216 vertices, ~521 M directed edges): BFS levels from the max-degree
+  vertex and CC labels bit-exact against the oracle; PageRank within L1 1e-6
+  of the FP64 oracle over two iterations, and the 20-iteration ranks against
+  two size-independent properties of the recurrence (the closed-form rank sum
+  and the isolated vertices' teleport value).
+- 4096 x 4096 grid (weights 1..1000, seed 2): SSSP from vertex 0 bit-exact
+  against the oracle's Dijkstra.
+- RMAT scale 22 (weights 1..255, seed 3), served: every batched result equal
+  to the same query run alone.
+
+The graphs come from the device generators, which other tests pin to the
+oracle's generators; the oracle runs on the CSR copied back to the host.
+"""
+import numpy as np
+import pytest
+
+import grus_framework_amd as grus
+
+INF = grus.INFINIT
+ALPHA = 0.85
+PR_TOL = 1e-6
+
+
+def host_csr(g):
+    from oracle import oracle_py as O
+    off, col, w = g.csr()
+    return O.CSR(off, col, w)
+
+
+def pagerank_sum(V, n_isolated, iters, alpha=ALPHA):
+    """Sum of the ranks after `iters` steps of
+    r_{k+1}(v) = (1-a)/V + a * sum_{u in N(v)} r_k(u)/deg(u), r_0 = (1-a)/V,
+    on a symmetric graph: each vertex with neighbours hands out exactly its
+    rank, isolated vertices hand out nothing and stay at (1-a)/V, so
+    S_{k+1} = (1-a) + a * (S_k - n_isolated * (1-a)/V)."""
+    base = (1.0 - alpha) / V
+    s = 1.0 - alpha
+    for _ in range(iters):
+        s = (1.0 - alpha) + alpha * (s - n_isolated * base)
+    return s
+
+
+def test_pagerank_sum_closed_form():
+    # the recurrence on a 4-cycle (no isolated vertex) keeps every rank at
+    # (1-a)/V + a*(1-a)/V after one step: sum = (1-a)(1+a)
+    assert abs(pagerank_sum(4, 0, 1) - (1 - ALPHA) * (1 + ALPHA)) < 1e-15
+    # all vertices isolated: the sum never moves from 1-a
+    assert abs(pagerank_sum(7, 7, 20) - (1 - ALPHA)) < 1e-15
+
+
+@pytest.fixture(scope="module")
+def s24(ctx):
+    print("rmat s24: generate", flush=True)
+    g = grus.Graph.rmat(ctx, 24)
+    print("rmat s24: copy the CSR to the host", flush=True)
+    h = host_csr(g)
+    yield g, h
+    g.close()
+
+
+@pytest.mark.gpu
+def test_rmat_s24_bfs_max_degree_source(s24, oracle):
+    g, h = s24
+    s = g.info.max_degree_vertex
+    deg = h.degrees()
+    assert s == int(np.argmax(deg))
+    print("s24 bfs src", s, flush=True)
+    lv, st = grus.bfs(g, s)
+    print("s24 bfs oracle", flush=True)
+    ref = oracle.bfs(h, s)
+    assert (lv == ref).all()
+    reached = ref != INF
+    assert st.vertices_reached == reached.sum()
+    assert st.edges_reached == deg[reached].sum()
+    assert st.pull_iterations > 0
+
+
+@pytest.mark.gpu
+def test_rmat_s24_cc(s24, oracle):
+    g, h = s24
+    print("s24 cc", flush=True)
+    lb, _ = grus.cc(g)
+    print("s24 cc oracle", flush=True)
+    assert (lb == oracle.cc(h)).all()
+
+
+@pytest.mark.gpu
+def test_rmat_s24_pagerank(s24, oracle):
+    g, h = s24
+    deg = h.degrees()
+    iso = deg == 0
+    assert iso.any()  # RMAT s24 leaves many vertices without edges
+    print("s24 pagerank, 2 iterations", flush=True)
+    r2, _ = grus.pagerank(g, ALPHA, 2)
+    print("s24 pagerank oracle, 2 iterations", flush=True)
+    assert np.abs(r2 - oracle.pagerank(h, ALPHA, 2)).sum() <= PR_TOL
+    print("s24 pagerank, 20 iterations", flush=True)
+    r20, _ = grus.pagerank(g, ALPHA, 20)
+    want = pagerank_sum(g.V, int(iso.sum()), 20)
+    assert abs(r20.astype(np.float64).sum() - want) <= PR_TOL
+    base = (1.0 - ALPHA) / g.V
+    assert np.abs(r20[iso].astype(np.float64) - base).max() <= 1e-6 * base
+
+
+@pytest.mark.gpu
+def test_grid_4096_sssp_from_vertex_0(ctx, oracle):
+    with grus.Graph.grid(ctx, 4096, 4096, weights=(1, 1000), wseed=2) as g:
+        assert (g.V, g.E) == (16777216, 67092480)
+        h = host_csr(g)
+        print("grid 4096 sssp", flush=True)
+        d, st = grus.sssp(g, 0)
+        print("grid 4096 dijkstra", flush=True)
+        assert (d == oracle.sssp(h, 0)).all()
+        assert st.vertices_reached == g.V
+
+
+@pytest.mark.gpu
+def test_rmat_s22_served_equals_alone(ctx):
+    with grus.Graph.rmat(ctx, 22, weights=(1, 255), wseed=3) as g:
+        off, _, _ = g.csr()
+        nz = np.nonzero(np.diff(off.astype(np.int64)))[0]
+        rng = np.random.default_rng(5)
+        qs = [(int(rng.integers(0, 2)), int(rng.choice(nz))) for _ in range(160)]
+        print("s22 single queries", flush=True)
+        alone = {}
+        for kind, s in qs:
+            (grus.bfs if kind == grus.Q_BFS else grus.sssp)(g, s, copy=False)
+            alone[(kind, s)] = g.digest()
+        print("s22 served", flush=True)
+        with grus.Server(g) as srv:
+            ids = {srv.submit(kind, s): (kind, s) for kind, s in qs}
+            got = [srv.poll(120000) for _ in qs]
+        assert all(r.status == 0 for r in got)
+        for r in got:
+            assert (r.reached, r.checksum) == alone[ids[r.id]], ids[r.id]
