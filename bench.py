@@ -454,7 +454,9 @@ def run_serving(grus):
 
     # the library's default policy: four engines per kind, a batch admitted
     # into an idle engine; and lane-by-lane admission into 8 engines per kind
+    REGIONS.begin("serving_rmat_s22_saturation_query")
     qps = saturate()
+    REGIONS.end(N)
     qps_cont = saturate(max_inflight_batches=8, admit_min_free=1)
     res["batched_qps"] = round(qps, 1)
     res["speedup_vs_serial"] = round(qps / serial_qps, 2)
