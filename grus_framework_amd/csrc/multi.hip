@@ -321,35 +321,67 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t 
   }
 }
 
+// A workgroup owns a contiguous span of `items` x BLOCK vertices (items <= 32,
+// one bit per item in `bits`): it commits the next bits of its span, counts
+// its new frontier vertices, reserves their queue slots with one atomic and
+// then writes them.  One queue atomic per workgroup instead of one per
+// workgroup-wide step of a grid-stride loop: the single counter address is
+// the serial point of the pass.
 __global__ __launch_bounds__(BLOCK) void k_sssp_finalize(const uint64_t *off, uint64_t *front,
-                                                         uint64_t *next, uint64_t V,
+                                                         uint64_t *next, uint64_t V, uint32_t items,
                                                          uint64_t running, uint32_t *queue,
                                                          MsCounters *cnt) {
   __shared__ unsigned long long s_alive;
-  __shared__ uint32_t s_app[BLOCK / WAVE + 1];
+  __shared__ uint32_t s_wpre[BLOCK / WAVE];
+  __shared__ uint32_t s_base;
   if (threadIdx.x == 0) s_alive = 0;
   __syncthreads();
-  uint64_t edges = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V; base += stride) {
-    uint64_t v = base + threadIdx.x;
-    uint64_t nw = 0;
+  const uint64_t first = (uint64_t)blockIdx.x * items * BLOCK + threadIdx.x;
+  uint32_t bits = 0;
+  uint64_t alive = 0, edges = 0;
+  for (uint32_t i = 0; i < items; i++) {
+    const uint64_t v = first + (uint64_t)i * BLOCK;
     if (v < V) {
-      uint64_t nx = next[v];
+      const uint64_t nx = next[v];
       if (nx) next[v] = 0;
-      nw = nx & running;
+      const uint64_t nw = nx & running;
       if (front[v] != nw) front[v] = nw;
-    }
-    const uint64_t wm = wave_or(nw);
-    if (lane_id() == 0 && wm) atomicOr(&s_alive, (unsigned long long)wm);
-    uint32_t slot = block_append(&cnt->qcount, nw != 0, s_app);
-    if (nw) {
-      queue[slot] = (uint32_t)v;
-      edges += off[v + 1] - off[v];
+      if (nw) {
+        bits |= 1u << i;
+        alive |= nw;
+        edges += off[v + 1] - off[v];
+      }
     }
   }
+  const uint64_t wm = wave_or(alive);
+  if (lane_id() == 0 && wm) atomicOr(&s_alive, (unsigned long long)wm);
+  // exclusive scan of the per-thread counts over the workgroup
+  const uint32_t c = __popc(bits);
+  uint32_t incl = c;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane_id() >= (uint32_t)o) incl += y;
+  }
+  const uint32_t wid = threadIdx.x / WAVE;
+  if (lane_id() == WAVE - 1) s_wpre[wid] = incl;
   __syncthreads();
-  if (threadIdx.x == 0 && s_alive) atomicOr((unsigned long long *)&cnt->alive, s_alive);
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < BLOCK / WAVE; w++) {
+      const uint32_t x = s_wpre[w];
+      s_wpre[w] = tot;
+      tot += x;
+    }
+    s_base = tot ? atomicAdd(&cnt->qcount, tot) : 0u;
+    if (s_alive) atomicOr((unsigned long long *)&cnt->alive, s_alive);
+  }
+  __syncthreads();
+  uint32_t slot = s_base + s_wpre[wid] + incl - c;
+  for (uint32_t m = bits; m; m &= m - 1) {
+    const uint32_t i = (uint32_t)(__ffs(m) - 1);
+    queue[slot++] = (uint32_t)(first + (uint64_t)i * BLOCK);
+  }
   block_add_u64(edges, (unsigned long long *)&cnt->front_edges);
 }
 
@@ -660,9 +692,14 @@ std::vector<int> MsEngine::step() {
           next_.as<uint64_t>(), vals_.as<uint32_t>());
       GRUS_LAUNCH_CHECK();
     }
-    ms::k_sssp_finalize<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-        out.off, front_.as<uint64_t>(), next_.as<uint64_t>(), V, running, queue_.as<uint32_t>(),
-        cnt);
+    // spans of up to 32 x BLOCK vertices, about 8 workgroups per CU
+    const uint64_t cap_v = (uint64_t)grid_cap(ctx, V, BLOCK, 8) * BLOCK;
+    const uint32_t items = (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, (V + cap_v - 1) / cap_v));
+    const uint64_t span = (uint64_t)items * BLOCK;
+    const unsigned fin_grid = (unsigned)std::max<uint64_t>(1, (V + span - 1) / span);
+    ms::k_sssp_finalize<<<fin_grid, BLOCK, 0, ctx.stream>>>(
+        out.off, front_.as<uint64_t>(), next_.as<uint64_t>(), V, items, running,
+        queue_.as<uint32_t>(), cnt);
     GRUS_LAUNCH_CHECK();
   }
   const MsCounters h = publish();
