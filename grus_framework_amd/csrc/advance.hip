@@ -38,40 +38,54 @@ __global__ void k_queue_to_bitmap(const uint32_t *q, uint32_t n, uint64_t *bm) {
   }
 }
 
+// A workgroup owns `items` x BLOCK consecutive bitmap words (items <=
+// B2Q_MAX_ITEMS; word i*BLOCK + t of the span goes to thread t, so every
+// round of loads is coalesced).  It scans the popcounts of the whole span and
+// reserves its queue slots with one atomic: the counter's single address is
+// the serial point of the pass, so the fewer workgroups the better once the
+// loads are in flight.
+constexpr uint32_t B2Q_MAX_ITEMS = 8;
+
 __global__ __launch_bounds__(BLOCK) void k_bitmap_to_queue(const uint64_t *bm, uint64_t W,
-                                                            uint32_t *q, uint32_t *count,
-                                                            uint64_t flip) {
+                                                            uint32_t items, uint32_t *q,
+                                                            uint32_t *count, uint64_t flip) {
   __shared__ uint32_t s_wave[BLOCK / WAVE + 1];
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  // block-uniform trip count; one lane per bitmap word
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < W; base += stride) {
-    uint64_t wd = base + threadIdx.x;
-    uint64_t m = wd < W ? bm[wd] ^ flip : 0ull;
-    uint32_t c = (uint32_t)__popcll(m);
-    uint32_t x = c;  // inclusive wave scan of the popcounts
+  const uint64_t first = (uint64_t)blockIdx.x * items * BLOCK + threadIdx.x;
+  uint64_t m[B2Q_MAX_ITEMS];
+  uint32_t c = 0;
 #pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-      uint32_t y = __shfl_up(x, o);
-      if ((int)lane_id() >= o) x += y;
+  for (uint32_t i = 0; i < B2Q_MAX_ITEMS; i++) {
+    const uint64_t wd = first + (uint64_t)i * BLOCK;
+    m[i] = (i < items && wd < W) ? bm[wd] ^ flip : 0ull;
+    c += (uint32_t)__popcll(m[i]);
+  }
+  uint32_t x = c;  // inclusive wave scan of the popcounts
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if ((int)lane_id() >= o) x += y;
+  }
+  const uint32_t wid = threadIdx.x / WAVE;
+  if (lane_id() == WAVE - 1) s_wave[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < BLOCK / WAVE; i++) {
+      uint32_t t = s_wave[i];
+      s_wave[i] = s;
+      s += t;
     }
-    const uint32_t wid = threadIdx.x / WAVE;
-    if (lane_id() == WAVE - 1) s_wave[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t s = 0;
-      for (uint32_t i = 0; i < BLOCK / WAVE; i++) {
-        uint32_t t = s_wave[i];
-        s_wave[i] = s;
-        s += t;
-      }
-      s_wave[BLOCK / WAVE] = s ? atomicAdd(count, s) : 0u;
-    }
-    __syncthreads();
-    uint32_t at = s_wave[BLOCK / WAVE] + s_wave[wid] + (x - c);
-    __syncthreads();
-    while (m) {
-      int b = __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
+    s_wave[BLOCK / WAVE] = s ? atomicAdd(count, s) : 0u;
+  }
+  __syncthreads();
+  uint32_t at = s_wave[BLOCK / WAVE] + s_wave[wid] + (x - c);
+#pragma unroll
+  for (uint32_t i = 0; i < B2Q_MAX_ITEMS; i++) {
+    uint64_t mm = m[i];
+    const uint64_t wd = first + (uint64_t)i * BLOCK;
+    while (mm) {
+      int b = __ffsll((unsigned long long)mm) - 1;
+      mm &= mm - 1;
       q[at++] = (uint32_t)(wd * 64 + b);
     }
   }
@@ -153,14 +167,25 @@ void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uin
   scan_excl_u64(ctx, w.deg.as<uint64_t>(), w.scan.as<uint64_t>(), (size_t)F + 1);
 }
 
+// about one workgroup per CU, up to B2Q_MAX_ITEMS words per thread
+static void launch_bitmap_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q,
+                                   uint32_t *count, uint64_t flip) {
+  const uint64_t per_round = (uint64_t)ctx.cu_count * BLOCK;
+  const uint32_t items = (uint32_t)std::min<uint64_t>(
+      B2Q_MAX_ITEMS, std::max<uint64_t>(1, (W + per_round - 1) / per_round));
+  const uint64_t span = (uint64_t)items * BLOCK;
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, (W + span - 1) / span);
+  k_bitmap_to_queue<<<grid, BLOCK, 0, ctx.stream>>>(bm, W, items, q, count, flip);
+}
+
 void bitmap_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q, uint32_t *count) {
-  k_bitmap_to_queue<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(bm, W, q, count, 0ull);
+  launch_bitmap_to_queue(ctx, bm, W, q, count, 0ull);
   GRUS_LAUNCH_CHECK();
 }
 
 void clear_bits_to_queue(Context &ctx, const uint64_t *bm, uint64_t W, uint32_t *q,
                          uint32_t *count) {
-  k_bitmap_to_queue<<<grid_cap(ctx, W, BLOCK), BLOCK, 0, ctx.stream>>>(bm, W, q, count, ~0ull);
+  launch_bitmap_to_queue(ctx, bm, W, q, count, ~0ull);
   GRUS_LAUNCH_CHECK();
 }
 
