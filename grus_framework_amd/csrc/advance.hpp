@@ -66,6 +66,8 @@ void reached_stats(Context &ctx, const uint64_t *off, const uint32_t *val, uint6
 
 constexpr int LB_CHUNK = 1024;          // edges per workgroup step (4 per lane)
 constexpr int LB_WIN = LB_CHUNK + 64;   // queue entries staged in LDS
+constexpr uint32_t LB_PER = LB_CHUNK / BLOCK;  // edges per thread per chunk
+static_assert(LB_CHUNK % BLOCK == 0 && LB_PER <= 32, "a chunk's generate bits fit a u32 per thread");
 
 // first index in [lo, hi) with a[idx] > key (a non-decreasing), hi if none
 __device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t lo, uint32_t hi,
@@ -114,12 +116,16 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
       }
     }
     __syncthreads();
-    for (uint64_t kb = c0; kb < c1; kb += blockDim.x) {
-      uint64_t k = kb + threadIdx.x;
-      bool valid = k < c1;
-      int gen = 0;
-      uint32_t dst = 0;
-      if (valid) {
+    // every edge of the chunk is updated first (LB_CHUNK / BLOCK per thread,
+    // in coalesced rounds); the generated vertices then take their queue
+    // slots with one reservation per queue for the whole chunk
+    uint32_t dsts[LB_PER];
+    uint32_t g_near = 0, g_far = 0;  // bit i: the thread's edge i generates into qn / qfar
+#pragma unroll
+    for (uint32_t i = 0; i < LB_PER; i++) {
+      const uint64_t k = c0 + (uint64_t)i * BLOCK + threadIdx.x;
+      dsts[i] = 0;
+      if (k < c1) {
         uint32_t j, u;
         uint64_t base, row;
         if (staged) {
@@ -134,20 +140,28 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
           row = g.off[u];
         }
         uint64_t e = row + (k - base);
-        dst = edge_col(g, e);
+        const uint32_t dst = edge_col(g, e);
         uint32_t val = 0;  // what the updater wrote (a distance), for the generator
         bool upd = job.update(u, dst, e, edge_w(g, e), &val);
-        gen = job.generate(upd, dst, val);
+        const int gen = job.generate(upd, dst, val);
+        dsts[i] = dst;
+        if (gen == 1) g_near |= 1u << i;
+        if (gen == 2) g_far |= 1u << i;
       }
-      uint32_t slot = block_append(&cnt->next_count, gen == 1, s_app);
-      if (gen == 1) {
-        qn[slot] = dst;
-        my_edges += job.out_degree(dst);
+    }
+    uint32_t slot = block_reserve(&cnt->next_count, (uint32_t)__popc(g_near), s_app);
+#pragma unroll
+    for (uint32_t i = 0; i < LB_PER; i++) {
+      if ((g_near >> i) & 1u) {
+        qn[slot++] = dsts[i];
+        my_edges += job.out_degree(dsts[i]);
       }
-      if (Job::kFar) {
-        uint32_t fslot = block_append(&cnt->far_count, gen == 2, s_app);
-        if (gen == 2) qfar[fslot] = dst;
-      }
+    }
+    if (Job::kFar) {
+      uint32_t fslot = block_reserve(&cnt->far_count, (uint32_t)__popc(g_far), s_app);
+#pragma unroll
+      for (uint32_t i = 0; i < LB_PER; i++)
+        if ((g_far >> i) & 1u) qfar[fslot++] = dsts[i];
     }
     __syncthreads();
   }

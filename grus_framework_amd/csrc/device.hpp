@@ -64,6 +64,36 @@ __device__ __forceinline__ uint32_t block_append(uint32_t *count, bool pred, uin
   return pred ? slot : 0xFFFFFFFFu;
 }
 
+// Reserves n slots per thread at *count with one atomic per workgroup and
+// returns the thread's first slot (a workgroup's slots are contiguous, in
+// thread order).  Every thread of the workgroup must call it; lds holds
+// blockDim.x / WAVE + 1 entries.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t *count, uint32_t n, uint32_t *lds) {
+  const uint32_t nwaves = blockDim.x / WAVE;
+  const uint32_t wid = threadIdx.x / WAVE;
+  uint32_t x = n;  // inclusive wave scan
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if ((int)lane_id() >= o) x += y;
+  }
+  if (lane_id() == WAVE - 1) lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < nwaves; i++) {
+      const uint32_t c = lds[i];
+      lds[i] = s;
+      s += c;
+    }
+    lds[nwaves] = s ? atomicAdd(count, s) : 0u;
+  }
+  __syncthreads();
+  const uint32_t slot = lds[nwaves] + lds[wid] + (x - n);
+  __syncthreads();  // lds reusable by the next call
+  return slot;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
