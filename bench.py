@@ -496,26 +496,40 @@ def run_serving(grus):
 
 
 def run_capped(grus, ctx):
-    """BFS + PageRank on RMAT s27: resident in HBM vs edges capped at 8 GB of HBM."""
-    res = {}
+    """BFS + PageRank (2 iterations) on RMAT s27: resident in HBM vs the whole
+    device capped at 8 GiB (graph part, app workspaces and every other buffer
+    of the library; most edges then stream from pinned host memory).  The
+    peak of the bytes the library held on the device during the capped apps
+    is reported beside the cap."""
+    cap = 8 << 30
+    res = {"device_cap_bytes": cap}
     for mode in ("resident", "capped"):
-        pl = grus.place("capped", hbm_cap_bytes=8 << 30) if mode == "capped" else None
+        pl = grus.place("capped", device_budget_bytes=cap) if mode == "capped" else None
         g = grus.Graph.rmat(ctx, 27, placement=pl)
-        src = g.info.max_degree_vertex
-        grus.bfs(g, src, copy=False)
-        _, st = grus.bfs(g, src, copy=False)
-        digest = g.digest()
-        _, sp = grus.pagerank(g, 0.85, 2, copy=False)
-        B = 2 * (8 * (g.V + 1) + 8 * g.E + 12 * g.V)
-        res[mode] = {"bfs_ms": round(st.device_ms, 2),
-                     "bfs_GTEPS": round(st.edges_reached / st.device_ms / 1e6, 2),
-                     "pagerank_2it_ms": round(sp.device_ms, 1),
-                     "pagerank_GBps": round(B / sp.device_ms / 1e6, 1),
-                     "hbm_graph_GB": round(g.info.device_bytes / 1e9, 2),
-                     "host_edges_GB": round(g.info.host_bytes / 1e9, 2), "bfs_digest": digest[1]}
-        if mode == "resident":
-            res["V"], res["E"] = int(g.V), int(g.E)
-        g.close()
+        try:
+            held, _ = ctx.device_memory(reset_peak=True)
+            src = g.info.max_degree_vertex
+            grus.bfs(g, src, copy=False)
+            _, st = grus.bfs(g, src, copy=False)
+            digest = g.digest()
+            _, sp = grus.pagerank(g, 0.85, 2, copy=False)
+            _, peak = ctx.device_memory()
+            B = 2 * (8 * (g.V + 1) + 8 * g.E + 12 * g.V)
+            res[mode] = {"bfs_ms": round(st.device_ms, 2),
+                         "bfs_GTEPS": round(st.edges_reached / st.device_ms / 1e6, 2),
+                         "pagerank_2it_ms": round(sp.device_ms, 1),
+                         "pagerank_GBps": round(B / sp.device_ms / 1e6, 1),
+                         "hbm_graph_GB": round(g.info.device_bytes / 1e9, 2),
+                         "host_edges_GB": round(g.info.host_bytes / 1e9, 2),
+                         "device_bytes_after_placement": int(held),
+                         "device_peak_bytes_during_apps": int(peak),
+                         "bfs_digest": digest[1]}
+            if mode == "resident":
+                res["V"], res["E"] = int(g.V), int(g.E)
+        finally:
+            g.close()
+            ctx.set_device_budget(0)
+    res["capped_within_cap"] = res["capped"]["device_peak_bytes_during_apps"] <= cap
     res["bfs_levels_identical"] = res["resident"].pop("bfs_digest") == res["capped"].pop("bfs_digest")
     log("capped s27: %s" % res)
     return res

@@ -36,10 +36,14 @@ def device_count():
     return n.value
 
 
-def place(mode="hbm", hbm_cap_bytes=0, pf=False, ab=False, rm=False, pl=False):
-    """Placement request; flag names follow include/app.cuh:27-31."""
+def place(mode="hbm", hbm_cap_bytes=0, pf=False, ab=False, rm=False, pl=False,
+          device_budget_bytes=0):
+    """Placement request; flag names follow include/app.cuh:27-31.
+    device_budget_bytes (capped mode) caps the whole device footprint: graph,
+    app workspaces and everything else the library holds on the device."""
     modes = {"hbm": PLACE_HBM, "managed": PLACE_MANAGED, "capped": PLACE_CAPPED}
-    return Place(modes[mode], int(hbm_cap_bytes), int(pf), int(ab), int(rm), int(pl))
+    return Place(modes[mode], int(hbm_cap_bytes), int(pf), int(ab), int(rm), int(pl),
+                 int(device_budget_bytes))
 
 
 # ------------------------------------------------------------------ .gr I/O
@@ -84,6 +88,17 @@ class Context:
     @tuning.setter
     def tuning(self, t):
         capi.check(_lib().grus_set_tuning(self.h, C.byref(t)))
+
+    def set_device_budget(self, nbytes):
+        """Cap the device bytes the library may hold (0 = no cap)."""
+        capi.check(_lib().grus_set_device_budget(self.h, int(nbytes)))
+
+    def device_memory(self, reset_peak=False):
+        """(bytes the library holds on the device, peak since the last reset)."""
+        held, peak = C.c_uint64(), C.c_uint64()
+        capi.check(_lib().grus_device_memory(self.h, C.byref(held), C.byref(peak),
+                                             int(reset_peak)))
+        return held.value, peak.value
 
     @property
     def cu_count(self):
@@ -302,21 +317,39 @@ class Server:
         h = C.c_void_p()
         capi.check(_lib().grus_server_create(g.h, C.byref(opts), C.byref(h)))
         self.h = h
+        self._outs = {}  # query id -> caller's result array, held until polled
 
     def submit(self, kind, src, out=None):
+        """Queue one query.  `out`, when given, receives the V-entry result
+        (copy_results servers): it must be a writable C-contiguous uint32
+        array of at least V entries, and it is kept alive here until poll()
+        returns its query, because a worker thread writes it asynchronously."""
+        if out is not None:
+            if not isinstance(out, np.ndarray) or out.dtype != np.uint32:
+                raise TypeError("out must be a numpy uint32 array")
+            if not out.flags.c_contiguous or not out.flags.writeable:
+                raise ValueError("out must be writable and C-contiguous")
+            if out.size < self.g.V:
+                raise ValueError("out holds %d entries, the graph has %d vertices"
+                                 % (out.size, self.g.V))
         qid = C.c_uint64()
         capi.check(_lib().grus_server_submit(self.h, kind, src, _p(out), C.byref(qid)))
+        if out is not None:
+            self._outs[qid.value] = out
         return qid.value
 
     def poll(self, timeout_ms=-1):
         r = capi.QueryResult()
         capi.check(_lib().grus_server_poll(self.h, C.byref(r), timeout_ms))
+        self._outs.pop(r.id, None)
         return r
 
     def close(self):
         if self.h:
+            # the destroy joins the workers: no result is written afterwards
             capi.check(_lib().grus_server_destroy(self.h))
             self.h = None
+            self._outs.clear()
 
     def __enter__(self):
         return self

@@ -24,7 +24,7 @@ TUNE_BFS, TUNE_SSSP, TUNE_PR = 0, 1, 2
 class Place(C.Structure):
     _fields_ = [("mode", C.c_int), ("hbm_cap_bytes", C.c_uint64), ("prefetch", C.c_int),
                 ("accessed_by", C.c_int), ("read_mostly", C.c_int),
-                ("preferred_location", C.c_int)]
+                ("preferred_location", C.c_int), ("device_budget_bytes", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -81,6 +81,8 @@ SIGNATURES = [
     ("grus_set_tuning", _i, [_vp, C.POINTER(Tuning)]),
     ("grus_set_cu_mask", _i, [_vp, C.POINTER(C.c_uint32), C.c_uint32]),
     ("grus_cu_count", _i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    ("grus_set_device_budget", _i, [_vp, _u64]),
+    ("grus_device_memory", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64), _i]),
     ("grus_gr_header", _i, [C.c_char_p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i)]),
     ("grus_gr_read", _i, [C.c_char_p, _vp, _vp, _vp]),
     ("grus_gr_write", _i, [C.c_char_p, _u64, _u64, _vp, _vp, _vp]),

@@ -94,10 +94,12 @@ void usage(const char *prog) {
           "  reference flags: --output FILE --src N --device N --ngpu 1 --pull\n"
           "                   --opt --pf --ab --rm --pl --wl_th X (accepted, unused)\n"
           "  placement:       --placement hbm|managed|capped --hbm_cap_gb G\n"
+          "                   --mem_limit_gb G (capped: whole-device cap)\n"
           "  co-location:     --cu_limit N (run on the first N compute units)\n"
           "  inputs:          --symmetric --edge_factor N --seed N --weights LO,HI --wseed N\n"
           "  apps:            --direction auto|push|pull (bfs)  --iters N --alpha A (pr)\n"
-          "                   --method hook|lp (cc)  --pr_unnormalized (dump ranks x V)\n",
+          "                   --method hook|lp (cc)  --pr_unnormalized (dump ranks x V)\n"
+          "  output:          --precise_output (ranks with 9 digits instead of 6)\n",
           prog);
 }
 
@@ -130,6 +132,9 @@ int main(int argc, char **argv) {
   else if (pm == "capped") place.mode = GRUS_PLACE_CAPPED;
   else place.mode = GRUS_PLACE_HBM;
   place.hbm_cap_bytes = (uint64_t)(f.dbl("hbm_cap_gb", 0.0) * (1ull << 30));
+  // --mem_limit_gb: the whole device footprint, as the reference's
+  // tools/mem_limit.cu caps the device before a run (capped placement)
+  place.device_budget_bytes = (uint64_t)(f.dbl("mem_limit_gb", 0.0) * (1ull << 30));
   place.prefetch = f.flag("pf", false);
   place.accessed_by = f.flag("ab", false);
   place.read_mostly = f.flag("rm", false);
@@ -220,9 +225,13 @@ int main(int argc, char **argv) {
     }
     printf("Saving the results into the following file:\n>> %s\n", path.c_str());
     double scale = f.flag("pr_unnormalized", false) ? (double)info.V : 1.0;
+    // ranks as the reference's SaveResults streams them (an ofstream at its
+    // default precision: %g, 6 significant digits, include/print.cuh:36-45);
+    // --precise_output keeps 9 digits, enough to round-trip an FP32 value
+    const char *ffmt = f.flag("precise_output", false) ? "%llu %.9g\n" : "%llu %g\n";
     for (uint64_t i = 0; i < info.V; i++) {
       if (!u32.empty()) fprintf(out, "%llu %u\n", (unsigned long long)i, u32[i]);
-      else fprintf(out, "%llu %.9g\n", (unsigned long long)i, f32[i] * scale);
+      else fprintf(out, ffmt, (unsigned long long)i, (double)(float)(f32[i] * scale));
     }
     fclose(out);
     printf("Done saving.\n");

@@ -389,6 +389,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
   ctx.activate();
   if (src >= g.V) raise(GRUS_E_INVALID, "source vertex out of range");
   if (dir < GRUS_DIR_AUTO || dir > GRUS_DIR_PULL) raise(GRUS_E_INVALID, "direction mode");
+  g.make_room(Graph::App::BFS);
   if (!g.bfs) g.bfs = std::make_shared<BfsWork>();
   BfsWork &w = *g.bfs;
   w.ensure(g.V);

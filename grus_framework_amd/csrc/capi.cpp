@@ -110,6 +110,23 @@ int grus_cu_count(grus_ctx *ctx, int *active, int *total) {
   });
 }
 
+int grus_set_device_budget(grus_ctx *ctx, uint64_t bytes) {
+  return guard([&] {
+    need(ctx, "ctx");
+    mgg::devmem_set_budget(ctx->c->device, bytes);
+  });
+}
+
+int grus_device_memory(grus_ctx *ctx, uint64_t *held, uint64_t *peak, int reset) {
+  return guard([&] {
+    need(ctx, "ctx");
+    const int d = ctx->c->device;
+    if (held) *held = mgg::devmem_live(d);
+    if (peak) *peak = mgg::devmem_peak(d);
+    if (reset) mgg::devmem_reset_peak(d);
+  });
+}
+
 int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in) {
   return guard([&] {
     need(ctx, "ctx");

@@ -138,6 +138,7 @@ void run_cc(Graph &g, int method, grus_stats *st) {
   if (method != 0 && method != 1) raise(GRUS_E_INVALID, "cc method");
   if (!g.symmetric)
     raise(GRUS_E_UNSUPPORTED, "connected components need a symmetric graph (weak CC)");
+  g.make_room(Graph::App::CC);
   if (!g.cc) g.cc = std::make_shared<CcWork>();
   CcWork &w = *g.cc;
   const uint64_t V = g.V;

@@ -6,6 +6,7 @@
 // scan), so RMAT scale 24 is built in well under a second instead of the
 // reference's host-side serial loops (csr_tocsc, include/graph.cuh:273-303).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 
 #include "device.hpp"
@@ -15,11 +16,45 @@
 
 namespace mgg {
 
+// ------------------------------------------------------------------ accounting
+namespace {
+constexpr int MAX_DEV = 64;
+std::atomic<uint64_t> g_live[MAX_DEV], g_peak[MAX_DEV], g_budget[MAX_DEV];
+int dev_slot(int dev) {
+  if (dev < 0 || dev >= MAX_DEV) raise(GRUS_E_NODEVICE, "device index out of range");
+  return dev;
+}
+}  // namespace
+
+uint64_t devmem_live(int dev) { return g_live[dev_slot(dev)].load(); }
+uint64_t devmem_peak(int dev) { return g_peak[dev_slot(dev)].load(); }
+void devmem_reset_peak(int dev) { g_peak[dev_slot(dev)].store(g_live[dev].load()); }
+uint64_t devmem_budget(int dev) { return g_budget[dev_slot(dev)].load(); }
+void devmem_set_budget(int dev, uint64_t bytes) {
+  const int d = dev_slot(dev);
+  if (bytes && g_live[d].load() > bytes)
+    raise(GRUS_E_BUDGET, "device budget below the bytes the library already holds (" +
+                             std::to_string(g_live[d].load()) + ")");
+  g_budget[d].store(bytes);
+}
+
+uint64_t workspace_reserve(uint64_t V, uint64_t E) {
+  // per vertex, the largest app: SSSP's two near and two far queues and two
+  // stamp arrays (24 B), the push operator's degree scan (16 B), the result
+  // (4 B) and bitmaps; BFS needs 42 B, PageRank's pull 12 B, CC 30 B.  Per
+  // 2048-edge window, the pull rounds' window tables (12 B).  Plus control
+  // blocks, rocPRIM temporaries and the in-degree-0 bitmap.
+  return 48 * (V + 2) + 16 * (E / 2048 + 2) + (64ull << 20);
+}
+
 // ------------------------------------------------------------------ Buf
 void Buf::release() {
   if (!p) return;
   switch (kind) {
     case Mem::Device:
+      (void)hipFree(p);
+      if (dev >= 0 && dev < MAX_DEV) g_live[dev].fetch_sub(bytes);
+      break;
     case Mem::Managed:
       (void)hipFree(p);
       break;
@@ -38,9 +73,30 @@ void Buf::alloc(size_t nbytes, Mem k) {
   release();
   if (nbytes == 0) nbytes = 8;
   switch (k) {
-    case Mem::Device:
-      GRUS_HIP(hipMalloc(&p, nbytes));
+    case Mem::Device: {
+      int d = 0;
+      GRUS_HIP(hipGetDevice(&d));
+      d = dev_slot(d);
+      const uint64_t held = g_live[d].fetch_add(nbytes) + nbytes;
+      const uint64_t budget = g_budget[d].load();
+      if (budget && held > budget) {
+        g_live[d].fetch_sub(nbytes);
+        raise(GRUS_E_BUDGET, "device budget of " + std::to_string(budget) +
+                                 " bytes exceeded: " + std::to_string(held - nbytes) +
+                                 " held, " + std::to_string(nbytes) + " requested");
+      }
+      const hipError_t e = hipMalloc(&p, nbytes);
+      if (e != hipSuccess) {
+        g_live[d].fetch_sub(nbytes);
+        p = nullptr;
+        GRUS_HIP(e);
+      }
+      uint64_t pk = g_peak[d].load();
+      while (held > pk && !g_peak[d].compare_exchange_weak(pk, held)) {
+      }
+      dev = d;
       break;
+    }
     case Mem::Managed:
       GRUS_HIP(hipMallocManaged(&p, nbytes, hipMemAttachGlobal));
       break;
@@ -138,6 +194,17 @@ void Context::read_ctl(size_t n) {
 
 // ------------------------------------------------------------------ Graph
 Graph::Graph(Context *c) : ctx(c) {}
+
+void Graph::make_room(App keep) {
+  if (!devmem_budget(ctx->device)) return;  // unbudgeted: keep every cache
+  GRUS_HIP(hipStreamSynchronize(ctx->stream));
+  if (keep != App::BFS) bfs.reset();
+  if (keep != App::SSSP) sssp.reset();
+  // SSSP's binned pull rounds run on PageRank's layout (HBM graphs): kept
+  if (keep != App::PR && !(keep == App::SSSP && placement == GRUS_PLACE_HBM)) pr.reset();
+  if (keep != App::CC) cc.reset();
+  if (keep != App::MULTI) ms.reset();
+}
 
 CsrView Graph::csr() const {
   CsrView v;
@@ -613,6 +680,22 @@ void graph_apply_placement(Graph &g, const grus_place *place) {
   uint64_t per_edge = g.weighted ? 8 : 4;
   // always resident: the offsets and the first neighbour of every row
   uint64_t fixed = (g.V + 1) * 8 + g.V * 4;
+  const uint64_t budget = place->device_budget_bytes;
+  if (budget) {
+    // whole-device cap (the reference caps the device, tools/mem_limit.cu:6-18,
+    // and places from what cudaMemGetInfo leaves, include/graph.cuh:171-205):
+    // the graph gets what the budget leaves after the bytes the library holds
+    // for other purposes and the largest app workspace
+    ctx->scan_tmp.release();  // the build's sort / scan temporaries
+    const uint64_t mine = g.off.bytes + g.col.bytes + g.w.bytes + g.first_col.bytes;
+    const uint64_t others = devmem_live(ctx->device) - mine;
+    const uint64_t need = others + fixed + workspace_reserve(g.V, g.E);
+    if (budget < need)
+      raise(GRUS_E_BUDGET, "device budget below the offsets, first neighbours and app "
+                           "workspaces (" + std::to_string(need) + " bytes)");
+    const uint64_t room = budget - need + fixed;
+    cap = cap ? std::min(cap, room) : room;
+  }
   if (cap && cap < fixed)
     raise(GRUS_E_BUDGET, "HBM cap below the offsets and first-neighbour arrays (12 bytes per vertex)");
   g.first_col.alloc(std::max<uint64_t>(g.V, 1) * 4);
@@ -646,6 +729,8 @@ void graph_apply_placement(Graph &g, const grus_place *place) {
   shrink(g.col, split);
   if (g.weighted) shrink(g.w, split);
   g.host_bytes = far * per_edge;
+  // from here on every device allocation counts against the budget
+  if (budget) devmem_set_budget(ctx->device, budget);
 }
 
 }  // namespace mgg

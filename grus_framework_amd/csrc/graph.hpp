@@ -18,11 +18,25 @@ namespace mgg {
 
 enum class Mem { None, Device, Managed, HostPinned };
 
+// Library-wide accounting of device memory (hipMalloc'd Buf memory, per
+// device): bytes held now, the peak since the last reset, and an optional
+// budget.  With a budget set, an allocation that would take the bytes held
+// above it fails with GRUS_E_BUDGET instead of reaching the device -- the
+// whole-device cap of the reference's memory-limit tool (tools/mem_limit.cu:
+// 6-18), counted per allocation instead of by a balloon.  Managed and pinned
+// host memory are not counted.
+uint64_t devmem_live(int dev);
+uint64_t devmem_peak(int dev);
+void devmem_reset_peak(int dev);
+uint64_t devmem_budget(int dev);
+void devmem_set_budget(int dev, uint64_t bytes);  // 0 = no budget
+
 // Owning untyped buffer.
 struct Buf {
   void *p = nullptr;
   size_t bytes = 0;
   Mem kind = Mem::None;
+  int dev = -1;  // device of a Mem::Device allocation (for the accounting)
   Buf() = default;
   Buf(const Buf &) = delete;
   Buf &operator=(const Buf &) = delete;
@@ -33,9 +47,11 @@ struct Buf {
       p = o.p;
       bytes = o.bytes;
       kind = o.kind;
+      dev = o.dev;
       o.p = nullptr;
       o.bytes = 0;
       o.kind = Mem::None;
+      o.dev = -1;
     }
     return *this;
   }
@@ -140,6 +156,11 @@ struct Graph {
 
   explicit Graph(Context *c);
   ~Graph();
+  // Under a device budget, an app first releases the workspaces other apps
+  // left cached on this graph, so that the budget has to hold the largest
+  // app's workspace, not their sum (workspace_reserve sizes that).
+  enum class App { BFS, SSSP, PR, CC, MULTI };
+  void make_room(App keep);
   CsrView csr() const;
   CsrView csc();  // in-edges; aliases csr() when symmetric
   uint64_t words() const { return (V + 63) / 64; }
@@ -163,6 +184,10 @@ void csr_from_sorted_keys(Context &ctx, const uint64_t *keys, uint64_t m, uint64
                           uint64_t *off, uint32_t *col);
 // (re)place a device-built CSR per the placement request
 void graph_apply_placement(Graph &g, const grus_place *place);
+// Upper bound of the device bytes one app (BFS, SSSP, PageRank pull, CC) on
+// a capped graph of V vertices and E edges allocates beside the graph: its
+// workspace, the result array, control blocks and scan temporaries.
+uint64_t workspace_reserve(uint64_t V, uint64_t E);
 
 // .gr I/O (gr_io.cpp; host only)
 struct GrHeader {

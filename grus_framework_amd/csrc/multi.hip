@@ -786,6 +786,7 @@ namespace {
 void run_multi(Graph &g, MsKind kind, const uint32_t *srcs, int k, uint32_t *out,
                grus_stats *st) {
   if (k < 1 || k > 64) raise(GRUS_E_INVALID, "between 1 and 64 sources per batch");
+  g.make_room(Graph::App::MULTI);
   if (kind == MsKind::BFS) {
     if (!g.ms || g.ms->kind() != MsKind::BFS) g.ms = std::make_shared<MsEngine>(g, kind, 64, false);
   } else {

@@ -1608,6 +1608,7 @@ void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st) {
   if (iters < 0) raise(GRUS_E_INVALID, "iteration count");
   if (!(alpha >= 0.0f && alpha <= 1.0f)) raise(GRUS_E_INVALID, "damping factor");
   if (g.V == 0) raise(GRUS_E_INVALID, "empty graph");
+  g.make_room(Graph::App::PR);
   if (!g.pr) g.pr = std::make_shared<PrWork>();
   PrWork &w = *g.pr;
   const uint64_t V = g.V;

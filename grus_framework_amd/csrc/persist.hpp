@@ -62,12 +62,20 @@ __device__ __forceinline__ bool grid_sync(GridSync *gs, unsigned int nblocks, un
 // st_agent, atomicMin/Exch/Add), which are coherent across the XCDs' L2s;
 // read-only graph arrays are read normally.  An agent-scope fence, by
 // contrast, writes back and invalidates an XCD's whole L2 and costs tens of
-// microseconds per round when every workgroup issues one.  __syncthreads
-// waits for this workgroup's outstanding memory operations before thread 0
-// arrives.
+// microseconds per round when every workgroup issues one.
+//
+// __syncthreads alone does not order them: for gfx950 it compiles to
+// s_barrier without waiting on the vector memory counter, so a wave's
+// agent-scope stores (the queue entries written by publish()) may still be
+// in flight when thread 0's arrival completes the barrier for another
+// workgroup.  Every wave therefore waits for all of its outstanding memory
+// operations (s_waitcnt with every counter at zero) before the workgroup
+// barrier; a completed agent-scope store is visible to the agent-scope
+// loads other workgroups issue after the grid barrier.
 __device__ __forceinline__ bool grid_sync_atomic(GridSync *gs, unsigned int nblocks,
                                                  unsigned int &gen, uint64_t timeout) {
   __shared__ unsigned int s_abort;
+  __builtin_amdgcn_s_waitcnt(0);  // vmcnt = expcnt = lgkmcnt = 0
   __syncthreads();
   if (threadIdx.x == 0) {
     // monotonic arrival count (never reset): barrier number `gen` completes

@@ -202,6 +202,7 @@ class ServerImpl : public Server {
     }
     // queries still held by this engine or waiting for its kind fail
     std::vector<Query> orphans;
+    int orphan_status;
     {
       std::lock_guard<std::mutex> lk(mu_);
       for (int i = 0; i < 64; i++)
@@ -210,8 +211,10 @@ class ServerImpl : public Server {
         for (auto &q : pending_[kind]) orphans.push_back(q);
         pending_[kind].clear();
       }
+      // read under the lock: the destructor and fail() write them under it
+      orphan_status = stop_ ? GRUS_E_CLOSED : failed_code_;
     }
-    for (auto &q : orphans) finish(q, stop_ ? GRUS_E_CLOSED : failed_code_, 0, 0);
+    for (auto &q : orphans) finish(q, orphan_status, 0, 0);
   }
 
   void fail(int code, const std::string &msg) {

@@ -561,6 +561,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   Context &ctx = *g.ctx;
   ctx.activate();
   if (src >= g.V) raise(GRUS_E_INVALID, "source vertex out of range");
+  g.make_room(Graph::App::SSSP);
   if (!g.sssp) g.sssp = std::make_shared<SsspWork>();
   SsspWork &w = *g.sssp;
   w.ensure(ctx, g.V);

@@ -68,6 +68,17 @@ typedef struct grus_place {
   int accessed_by;        /* MANAGED: --ab  hipMemAdviseSetAccessedBy */
   int read_mostly;        /* MANAGED: --rm  hipMemAdviseSetReadMostly */
   int preferred_location; /* MANAGED: --pl  hipMemAdviseSetPreferredLocation */
+  uint64_t device_budget_bytes; /* CAPPED: cap of the whole device footprint
+                             (0 = none).  The graph's HBM part is sized so that
+                             it, every other device buffer the library holds
+                             and the largest app workspace fit under it, and
+                             the budget is then installed for the device as
+                             grus_set_device_budget does (the reference caps
+                             the whole device, tools/mem_limit.cu:6-18, and
+                             places from cudaMemGetInfo, include/graph.cuh:
+                             171-205).  The device build of a generated graph
+                             and its temporaries come before the cap, as the
+                             reference's loader fills host memory first. */
 } grus_place;
 
 /* Direction policy for BFS (samples/bfs/bfs.cu:135-173 push, :104-134 pull) */
@@ -164,6 +175,17 @@ int grus_set_tuning(grus_ctx *ctx, const grus_tuning *in);
 int grus_set_cu_mask(grus_ctx *ctx, const uint32_t *mask, uint32_t mask_words);
 /* CUs the context may use and the device's total. */
 int grus_cu_count(grus_ctx *ctx, int *active, int *total);
+/* Cap the device memory the library may hold on the context's device
+ * (0 = no cap): an allocation past it fails with GRUS_E_BUDGET, and under a
+ * cap every app releases the workspaces other apps left cached on its graph.
+ * Replaces the reference's memory-limit tool, a balloon cudaMalloc of all
+ * but X GB (tools/mem_limit.cu:6-18).  Fails with GRUS_E_BUDGET when the
+ * library already holds more. */
+int grus_set_device_budget(grus_ctx *ctx, uint64_t bytes);
+/* Device bytes the library holds on the context's device (every context
+ * and graph of the process), and the peak since the last reset; reset != 0
+ * restarts the peak from the bytes held now.  No reference counterpart. */
+int grus_device_memory(grus_ctx *ctx, uint64_t *held, uint64_t *peak, int reset);
 
 /* ---- host-side .gr I/O (no device needed) ----
  * Galois .gr v1: u64 header {1, sizeEdgeTy, N, M}, u64[N] end offsets,

@@ -19,6 +19,10 @@
  *                     and by the independent GPU generator.
  *   or_bfs            closed forms (path, star, cycle, complete, grid
  *                     corner = r+c) and brute-force all-pairs on tiny graphs.
+ *   or_bfs_certify    accepts or_bfs's levels and scipy's hop distances on
+ *                     RMAT and grids; rejects planted mistakes (a level one
+ *                     too high or too low, a second level-0 vertex, a wrong
+ *                     source, an unreached vertex beside a reached one).
  *   or_sssp           closed forms (unit-weight grid = Manhattan distance),
  *                     Floyd-Warshall on <= 64 vertices, O(E) certificate.
  *   or_cc             invariants (label constant on edges, label<=v,
@@ -71,6 +75,8 @@ uint64_t or_fnv64(const void *p, uint64_t nbytes, uint64_t h);
 /* ---- apps ---- */
 /* BFS levels: samples/bfs/bfs.cu:18-23 (init), :51-60 (updater: level =
  * iteration+1 on first discovery).  Unreached = OR_INFINIT. */
+int or_bfs_certify(uint64_t V, const uint64_t *off, const uint32_t *col, uint32_t src,
+                   const uint32_t *level, uint64_t *bad);
 void or_bfs(uint64_t V, const uint64_t *off, const uint32_t *col, uint32_t src,
             uint32_t *level);
 /* SSSP distances: samples/sssp/sssp.cu:18-23, :51-60 (relax d[v] >

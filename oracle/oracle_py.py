@@ -53,6 +53,8 @@ def lib():
         L.or_fnv64.restype = C.c_uint64
         L.or_fnv64.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.or_bfs.argtypes = [C.c_uint64, _u64p, _u32p, C.c_uint32, _u32p]
+        L.or_bfs_certify.argtypes = [C.c_uint64, _u64p, _u32p, C.c_uint32, _u32p,
+                                     C.POINTER(C.c_uint64)]
         L.or_sssp.argtypes = [C.c_uint64, _u64p, _u32p, C.c_void_p, C.c_uint32, _u32p]
         L.or_cc.argtypes = [C.c_uint64, _u64p, _u32p, _u32p]
         L.or_pagerank.argtypes = [C.c_uint64, _u64p, _u32p, _u64p, C.c_double, C.c_int, _f64p]
@@ -160,6 +162,15 @@ def bfs(g, src):
     out = np.zeros(g.V, np.uint32)
     lib().or_bfs(g.V, g.off, g.col, src, out)
     return out
+
+
+def bfs_certify(g, src, level):
+    """(code, vertex): 0 when `level` are the hop distances from src on the
+    symmetric graph g (or_bfs_certify), else the failed condition."""
+    bad = C.c_uint64()
+    level = np.ascontiguousarray(level, np.uint32)
+    code = lib().or_bfs_certify(g.V, g.off, g.col, src, level, C.byref(bad))
+    return int(code), int(bad.value)
 
 
 def sssp(g, src, unit=False):

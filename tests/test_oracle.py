@@ -303,3 +303,85 @@ def test_generator_fixture_hashes(oracle):
                                 wseed=spec.get("wseed", 0))
         assert g.E == spec["E"], key
         assert "%016x" % g.fnv() == spec["fnv"], key
+
+
+# ---------------------------------------------------------------- baselines
+# The OpenMP baselines (oracle/baseline_omp.c) are what bench.py's
+# cpu_baseline leg times; they must compute the oracle's results, or the
+# GPU/CPU ratios compare different work.
+@pytest.fixture(scope="module")
+def baseline_graphs(oracle):
+    return [("rmat12", oracle.gen_rmat(12, weights=(1, 255), wseed=3)),
+            ("rmat14", oracle.gen_rmat(14, weights=(1, 255), wseed=3)),
+            ("grid300x200", oracle.gen_grid(300, 200, weights=(1, 1000), wseed=2))]
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_baselines_equal_oracle(oracle, baseline_graphs, threads):
+    for name, g in baseline_graphs:
+        deg = g.degrees()
+        srcs = [int(np.argmax(deg)), int(np.nonzero(deg)[0][len(np.nonzero(deg)[0]) // 3])]
+        if name.startswith("grid"):
+            srcs.append(0)
+        for s in srcs:
+            lv, _ = oracle.bl_bfs(g, s, threads=threads)
+            assert (lv == oracle.bfs(g, s)).all(), (name, s)
+            for delta in (64, 1000):
+                d, _ = oracle.bl_sssp(g, s, delta=delta, threads=threads)
+                assert (d == oracle.sssp(g, s)).all(), (name, s, delta)
+        lb, _ = oracle.bl_cc(g, threads=threads)
+        assert (lb == oracle.cc(g)).all(), name
+        for iters in (2, 20):
+            r, _ = oracle.bl_pagerank(g, 0.85, iters, threads=threads)
+            assert r.dtype == np.float32
+            assert np.abs(r.astype(np.float64) - oracle.pagerank(g, 0.85, iters)).sum() <= 1e-6, \
+                (name, iters)
+
+
+def test_baselines_on_edge_cases(oracle):
+    # an isolated source, a graph without edges, and two components
+    off, col, w = csr_from_edges(6, [(0, 1), (1, 2), (4, 5)], weights=[3, 4, 7])
+    g = oracle.CSR(off, col, w)
+    for s in (0, 3, 5):
+        assert (oracle.bl_bfs(g, s)[0] == oracle.bfs(g, s)).all()
+        assert (oracle.bl_sssp(g, s)[0] == oracle.sssp(g, s)).all()
+    assert (oracle.bl_cc(g)[0] == oracle.cc(g)).all()
+    assert list(oracle.bl_cc(g)[0]) == [0, 0, 0, 3, 4, 4]
+    e = oracle.CSR(np.zeros(5, np.uint64), np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert list(oracle.bl_bfs(e, 2)[0]) == [INF, INF, 0, INF]
+    assert list(oracle.bl_cc(e)[0]) == [0, 1, 2, 3]
+
+
+def test_bfs_certificate_accepts_distances_and_rejects_planted_mistakes(oracle):
+    g = oracle.gen_rmat(12)
+    A = sp.csr_matrix((np.ones(g.E), g.col.astype(np.int64), g.off.astype(np.int64)),
+                      shape=(g.V, g.V))
+    src = int(np.argmax(g.degrees()))
+    lv = oracle.bfs(g, src)
+    hop = csgraph.shortest_path(A, unweighted=True, indices=src)
+    assert (np.where(np.isinf(hop), INF, hop).astype(np.uint32) == lv).all()
+    assert oracle.bfs_certify(g, src, lv) == (0, 0)
+    grid = oracle.gen_grid(40, 30)
+    assert oracle.bfs_certify(grid, 0, oracle.bfs(grid, 0)) == (0, 0)
+    reached = np.nonzero((lv != INF) & (lv > 1))[0]
+    v = int(reached[len(reached) // 2])
+    for delta in (+1, -1):  # one level too high or too low at one vertex
+        bad = lv.copy()
+        bad[v] = int(lv[v]) + delta
+        assert oracle.bfs_certify(g, src, bad)[0] in (2, 3), delta
+    bad = lv.copy()
+    bad[v] = 0  # a second level-0 vertex
+    assert oracle.bfs_certify(g, src, bad) == (1, v)
+    other = int(np.nonzero(lv == 1)[0][0])
+    assert oracle.bfs_certify(g, other, lv)[0] == 1  # the wrong source
+    unreached = np.nonzero(lv == INF)[0]
+    assert len(unreached) > 0  # RMAT leaves isolated vertices
+    # an unreached vertex that is reached by mistake with some level: its
+    # (absent) neighbours cannot be its parent
+    bad = lv.copy()
+    bad[int(unreached[0])] = 3
+    assert oracle.bfs_certify(g, src, bad)[0] == 3
+    # a reached vertex wrongly left unreached: its neighbours see it
+    bad = lv.copy()
+    bad[v] = INF
+    assert oracle.bfs_certify(g, src, bad)[0] == 2

@@ -66,3 +66,40 @@ def test_capped_rmat_generator_and_errors(ctx, reference):
     with pytest.raises(grus.GrusError) as e:
         grus.Graph.rmat(ctx, 15, placement=grus.place("capped", hbm_cap_bytes=1024))
     assert e.value.code == capi.E_BUDGET
+
+
+def test_device_budget_caps_the_whole_footprint(ctx, reference):
+    # a whole-device budget (the reference's tools/mem_limit.cu): the graph
+    # takes what the budget leaves beside the app workspaces, every app runs
+    # within it, and allocations past it fail with GRUS_E_BUDGET
+    src, (off, col, w), ref = reference
+    V, E = len(off) - 1, len(col)
+    held0, _ = ctx.device_memory()
+    budget = held0 + (96 << 20)  # workspaces (~64 MB slack) + part of the edges
+    try:
+        pl = grus.place("capped", device_budget_bytes=budget)
+        with grus.Graph.from_csr(ctx, off, col, w, symmetric=True, placement=pl) as g:
+            info = g.info
+            assert 0 < info.host_bytes < E * 8  # split between HBM and host
+            held, _ = ctx.device_memory(reset_peak=True)
+            assert held <= budget
+            check_same(run_all(g, src), ref)
+            _, peak = ctx.device_memory()
+            assert peak <= budget
+            # the budget is installed for the device: a second copy of the
+            # graph does not fit beside the first
+            with pytest.raises(grus.GrusError) as e:
+                grus.Graph.from_csr(ctx, off, col, w, symmetric=True)
+            assert e.value.code == capi.E_BUDGET
+        # a budget below what the library already holds is refused
+        with pytest.raises(grus.GrusError) as e:
+            ctx.set_device_budget(1024)
+        assert e.value.code == capi.E_BUDGET
+    finally:
+        ctx.set_device_budget(0)
+    # too small for the offsets and the workspaces: refused at placement
+    with pytest.raises(grus.GrusError) as e:
+        grus.Graph.from_csr(ctx, off, col, w, symmetric=True,
+                            placement=grus.place("capped", device_budget_bytes=held0 + (1 << 20)))
+    assert e.value.code == capi.E_BUDGET
+    ctx.set_device_budget(0)
