@@ -54,6 +54,12 @@ class GraphInfo(C.Structure):
                 ("host_bytes", C.c_uint64)]
 
 
+class CsrViewC(C.Structure):
+    _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("w", C.c_void_p),
+                ("col_far", C.c_void_p), ("w_far", C.c_void_p), ("split", C.c_uint64),
+                ("V", C.c_uint64), ("E", C.c_uint64)]
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("max_inflight_batches", C.c_int), ("max_batch", C.c_int),
                 ("state_budget_bytes", C.c_uint64), ("copy_results", C.c_int),
@@ -93,6 +99,8 @@ SIGNATURES = [
     ("grus_graph_gen_grid", _i, [_vp, _u32, _u32, _u32, _u32, _u64, C.POINTER(Place), _pp]),
     ("grus_graph_info_get", _i, [_vp, C.POINTER(GraphInfo)]),
     ("grus_graph_copy_csr", _i, [_vp, _vp, _vp, _vp]),
+    ("grus_graph_view", _i, [_vp, _i, C.POINTER(CsrViewC)]),
+    ("grus_ctx_stream", _i, [_vp, _pp]),
     ("grus_graph_destroy", _i, [_vp]),
     ("grus_bfs", _i, [_vp, _u32, _i, _vp, C.POINTER(Stats)]),
     ("grus_sssp", _i, [_vp, _u32, _vp, C.POINTER(Stats)]),

@@ -162,9 +162,10 @@ unsigned push_grid(Context &ctx, uint64_t total) { return grid_cap(ctx, total, L
 void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uint32_t *q,
                        uint32_t F) {
   k_queue_degrees<<<grid_cap(ctx, (uint64_t)F + 1, BLOCK), BLOCK, 0, ctx.stream>>>(
-      off, q, F, w.deg.as<uint64_t>());
+      off, q, F, w.deg_ptr());
   GRUS_LAUNCH_CHECK();
-  scan_excl_u64(ctx, w.deg.as<uint64_t>(), w.scan.as<uint64_t>(), (size_t)F + 1);
+  if (w.tmp) scan_excl_u64_with(ctx, w.deg_ptr(), w.scan_ptr(), (size_t)F + 1, w.tmp, w.tmp_bytes);
+  else scan_excl_u64(ctx, w.deg_ptr(), w.scan_ptr(), (size_t)F + 1);
 }
 
 // about one workgroup per CU, up to B2Q_MAX_ITEMS words per thread

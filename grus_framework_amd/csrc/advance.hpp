@@ -269,7 +269,7 @@ void push_advance(Context &ctx, PushWork &w, const CsrView &g, const uint32_t *q
                   uint32_t *qfar = nullptr) {
   if (F == 0 || total == 0) return;
   k_push_lb<Job><<<push_grid(ctx, total), BLOCK, 0, ctx.stream>>>(
-      g, q, w.scan.as<uint64_t>(), F, total, job, qn, qfar, cnt);
+      g, q, w.scan_ptr(), F, total, job, qn, qfar, cnt);
   GRUS_LAUNCH_CHECK();
 }
 

@@ -13,6 +13,11 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st);
 void run_pagerank(Graph &g, float alpha, int iters, grus_stats *st);
 // samples/cc/cc.cu:124-190
 void run_cc(Graph &g, int method, grus_stats *st);
+// the same into the caller's device labels, with every other device buffer
+// carved from `scratch` (no allocation); GRUS_E_BUDGET when it is too small
+void run_cc_in(Graph &g, int method, uint32_t *labels_dev, void *scratch, uint64_t scratch_bytes,
+               grus_stats *st);
+uint64_t cc_scratch_bytes(Context &ctx, uint64_t V, int method);
 // batched multi-source (no reference counterpart); results at g.ms output
 void run_bfs_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);
 void run_sssp_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);

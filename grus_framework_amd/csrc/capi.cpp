@@ -77,7 +77,7 @@ int grus_init(int device, grus_ctx **out) {
     need(out, "out");
     *out = nullptr;
     auto c = std::make_unique<grus_ctx>();
-    c->c = std::make_unique<mgg::Context>(device);
+    c->c = std::make_unique<mgg::Context>(device);  // leaves `device` current
     *out = c.release();
   });
 }
@@ -231,6 +231,26 @@ int grus_graph_gen_grid(grus_ctx *ctx, uint32_t rows, uint32_t cols, uint32_t wl
     auto g = std::make_unique<grus_graph>();
     g->g = mgg::graph_gen_grid(ctx->c.get(), rows, cols, wlo, whi, wseed, place);
     *out = g.release();
+  });
+}
+
+int grus_graph_view(grus_graph *g, int in_edges, grus_csr_view *out) {
+  return guard([&] {
+    need(g, "g");
+    need(out, "out");
+    if (in_edges != 0 && in_edges != 1) mgg::raise(GRUS_E_INVALID, "in_edges is 0 or 1");
+    mgg::Graph &G = *g->g;
+    G.ctx->activate();
+    const mgg::CsrView v = in_edges ? G.csc() : G.csr();
+    *out = grus_csr_view{v.off, v.col, v.w, v.col_far, v.w_far, v.split, v.V, v.E};
+  });
+}
+
+int grus_ctx_stream(grus_ctx *ctx, void **hip_stream) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(hip_stream, "hip_stream");
+    *hip_stream = static_cast<void *>(ctx->c->stream);
   });
 }
 

@@ -16,12 +16,13 @@
 // method 1 -- the reference's min-label propagation through the push
 //   operator (advance.hpp), frontier deduplicated by round stamps.
 #include <algorithm>
-#include <unordered_map>
-#include <vector>
+#include <cstring>
+#include <string>
 
 #include "advance.hpp"
 #include "apps.hpp"
 #include "device.hpp"
+#include "prims.hpp"
 
 namespace mgg {
 
@@ -132,18 +133,22 @@ struct job_t {
 
 }  // namespace cc
 
-void run_cc(Graph &g, int method, grus_stats *st) {
+// The device buffers of one CC run beyond the graph and the labels.
+struct CcSpace {
+  uint32_t *sample = nullptr;           // method 0: kSample roots
+  uint32_t *q[2] = {nullptr, nullptr};  // method 1: frontier queues, V + 1 each
+  uint32_t *stamp = nullptr;            // method 1: round stamps, V + 1
+  uint32_t *stamp_id = nullptr;         // method 1: last stamp handed out (host)
+  PushWork *push = nullptr;             // method 1: the push operator's scans
+};
+
+namespace {
+constexpr uint32_t kSample = 1024;  // roots sampled to find the giant component
+static_assert(kSample * 4 <= 4096, "the sample is read back through the pinned control block");
+
+void cc_core(Graph &g, int method, uint32_t *label, CcSpace &sp, grus_stats *st) {
   Context &ctx = *g.ctx;
-  ctx.activate();
-  if (method != 0 && method != 1) raise(GRUS_E_INVALID, "cc method");
-  if (!g.symmetric)
-    raise(GRUS_E_UNSUPPORTED, "connected components need a symmetric graph (weak CC)");
-  g.make_room(Graph::App::CC);
-  if (!g.cc) g.cc = std::make_shared<CcWork>();
-  CcWork &w = *g.cc;
   const uint64_t V = g.V;
-  g.result.ensure(V * 4);
-  uint32_t *label = g.result.as<uint32_t>();
   const CsrView gr = g.csr();
   uint64_t launches = 0;
   uint32_t iterations = 0;
@@ -160,20 +165,22 @@ void run_cc(Graph &g, int method, grus_stats *st) {
       GRUS_LAUNCH_CHECK();
       launches += 2;
     }
-    // most frequent root among 1024 pseudo-random vertices
-    const uint32_t ns = 1024;
-    w.sample.ensure(ns * 4);
-    cc::k_sample<<<div_up(ns, BLOCK), BLOCK, 0, ctx.stream>>>(label, V, ns, 0x5EEDull,
-                                                             w.sample.as<uint32_t>());
+    // most frequent root among kSample pseudo-random vertices (ties: the
+    // smaller root), read back through the pinned control block
+    cc::k_sample<<<div_up(kSample, BLOCK), BLOCK, 0, ctx.stream>>>(label, V, kSample, 0x5EEDull,
+                                                                  sp.sample);
     GRUS_LAUNCH_CHECK();
-    std::vector<uint32_t> hs(ns);
-    GRUS_HIP(hipMemcpyAsync(hs.data(), w.sample.p, ns * 4, hipMemcpyDeviceToHost, ctx.stream));
+    GRUS_HIP(hipMemcpyAsync(ctx.h_ctl, sp.sample, kSample * 4, hipMemcpyDeviceToHost, ctx.stream));
     ctx.sync();
-    std::unordered_map<uint32_t, uint32_t> freq;
+    uint32_t hs[kSample];
+    memcpy(hs, ctx.h_ctl, sizeof hs);
+    std::sort(hs, hs + kSample);
     uint32_t giant = hs[0], best = 0;
-    for (uint32_t x : hs) {
-      uint32_t c = ++freq[x];
-      if (c > best || (c == best && x < giant)) best = c, giant = x;
+    for (uint32_t i = 0; i < kSample;) {
+      uint32_t j = i;
+      while (j < kSample && hs[j] == hs[i]) j++;
+      if (j - i > best) best = j - i, giant = hs[i];  // ascending: ties keep the smaller
+      i = j;
     }
     cc::k_link_rest<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(gr, kRounds,
                                                                                giant, label);
@@ -183,32 +190,23 @@ void run_cc(Graph &g, int method, grus_stats *st) {
     launches += 3;
     iterations = kRounds + 1;
   } else {
-    if (w.cap_v < V) {
-      w.q[0].alloc((V + 1) * 4);
-      w.q[1].alloc((V + 1) * 4);
-      w.stamp.alloc((V + 1) * 4);
-      fill_u32(ctx, w.stamp.as<uint32_t>(), V + 1, 0);
-      w.push.ensure(V);
-      w.cap_v = V;
-      w.stamp_id = 0;
-    }
     auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
     const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
-    cc::k_iota<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(w.q[0].as<uint32_t>(), V);
+    uint32_t &stamp_id = *sp.stamp_id;
+    cc::k_iota<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(sp.q[0], V);
     GRUS_LAUNCH_CHECK();
     uint64_t nf = V, mf = g.E;
     int cur = 0;
     while (nf) {
       if (iterations > V + 1) raise(GRUS_E_HIP, "label propagation did not converge");
-      if (w.stamp_id >= 0xFFFFFF00u) {
-        fill_u32(ctx, w.stamp.as<uint32_t>(), V + 1, 0);
-        w.stamp_id = 0;
+      if (stamp_id >= 0xFFFFFF00u) {
+        fill_u32(ctx, sp.stamp, V + 1, 0);
+        stamp_id = 0;
       }
-      GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
-      push_scan_degrees(ctx, w.push, gr.off, w.q[cur].as<uint32_t>(), (uint32_t)nf);
-      cc::job_t job{label, w.stamp.as<uint32_t>(), gr.off, ++w.stamp_id};
-      push_advance(ctx, w.push, gr, w.q[cur].as<uint32_t>(), (uint32_t)nf, mf, job,
-                   w.q[cur ^ 1].as<uint32_t>(), cnt);
+      fill_u64(ctx, reinterpret_cast<uint64_t *>(cnt), sizeof(StepCounters) / 8, 0);
+      push_scan_degrees(ctx, *sp.push, gr.off, sp.q[cur], (uint32_t)nf);
+      cc::job_t job{label, sp.stamp, gr.off, ++stamp_id};
+      push_advance(ctx, *sp.push, gr, sp.q[cur], (uint32_t)nf, mf, job, sp.q[cur ^ 1], cnt);
       launches += 4;
       ctx.read_ctl(2);
       nf = hc->next_count;
@@ -229,6 +227,93 @@ void run_cc(Graph &g, int method, grus_stats *st) {
     st->edges_reached = g.E;
     st->kernel_launches = launches;
   }
+}
+
+void cc_check(Graph &g, int method) {
+  g.ctx->activate();
+  if (method != 0 && method != 1) raise(GRUS_E_INVALID, "cc method");
+  if (!g.symmetric)
+    raise(GRUS_E_UNSUPPORTED, "connected components need a symmetric graph (weak CC)");
+}
+}  // namespace
+
+void run_cc(Graph &g, int method, grus_stats *st) {
+  cc_check(g, method);
+  g.make_room(Graph::App::CC);
+  if (!g.cc) g.cc = std::make_shared<CcWork>();
+  CcWork &w = *g.cc;
+  Context &ctx = *g.ctx;
+  const uint64_t V = g.V;
+  g.result.ensure(V * 4);
+  CcSpace sp;
+  if (method == 0) {
+    w.sample.ensure(kSample * 4);
+    sp.sample = w.sample.as<uint32_t>();
+  } else {
+    if (w.cap_v < V) {
+      w.q[0].alloc((V + 1) * 4);
+      w.q[1].alloc((V + 1) * 4);
+      w.stamp.alloc((V + 1) * 4);
+      fill_u32(ctx, w.stamp.as<uint32_t>(), V + 1, 0);
+      w.push.ensure(V);
+      w.cap_v = V;
+      w.stamp_id = 0;
+    }
+    sp.q[0] = w.q[0].as<uint32_t>();
+    sp.q[1] = w.q[1].as<uint32_t>();
+    sp.stamp = w.stamp.as<uint32_t>();
+    sp.stamp_id = &w.stamp_id;
+    sp.push = &w.push;
+  }
+  cc_core(g, method, g.result.as<uint32_t>(), sp, st);
+}
+
+uint64_t cc_scratch_bytes(Context &ctx, uint64_t V, int method) {
+  if (method == 0) return kSample * 4;
+  // queues and stamps (3 x (V + 1) u32), degree and scan (2 x (V + 2) u64),
+  // the scan's temporaries, each start aligned to 256 bytes
+  auto up = [](uint64_t x) { return (x + 255) & ~255ull; };
+  return 3 * up((V + 1) * 4) + 2 * up((V + 2) * 8) + up(scan_excl_u64_bytes(ctx, V + 1));
+}
+
+void run_cc_in(Graph &g, int method, uint32_t *label, void *scratch, uint64_t scratch_bytes,
+               grus_stats *st) {
+  cc_check(g, method);
+  Context &ctx = *g.ctx;
+  const uint64_t V = g.V;
+  const uint64_t need = cc_scratch_bytes(ctx, V, method);
+  if (need > scratch_bytes)
+    raise(GRUS_E_BUDGET, "connected components (method " + std::to_string(method) + ") on " +
+                             std::to_string(V) + " vertices need " + std::to_string(need) +
+                             " bytes of scratch; " + std::to_string(scratch_bytes) +
+                             " are available");
+  auto *base = static_cast<unsigned char *>(scratch);
+  auto up = [](uint64_t x) { return (x + 255) & ~255ull; };
+  CcSpace sp;
+  PushWork push;  // borrowed buffers only: its own Bufs stay empty
+  uint32_t stamp_id = 0;
+  if (method == 0) {
+    sp.sample = reinterpret_cast<uint32_t *>(base);
+  } else {
+    uint64_t at = 0;
+    auto take = [&](uint64_t bytes) {
+      unsigned char *p = base + at;
+      at += up(bytes);
+      return p;
+    };
+    sp.q[0] = reinterpret_cast<uint32_t *>(take((V + 1) * 4));
+    sp.q[1] = reinterpret_cast<uint32_t *>(take((V + 1) * 4));
+    sp.stamp = reinterpret_cast<uint32_t *>(take((V + 1) * 4));
+    push.deg_ext = reinterpret_cast<uint64_t *>(take((V + 2) * 8));
+    push.scan_ext = reinterpret_cast<uint64_t *>(take((V + 2) * 8));
+    push.tmp_bytes = up(scan_excl_u64_bytes(ctx, V + 1));
+    push.tmp = take(push.tmp_bytes);
+    // the scratch keeps the last call's contents: stamps start from zero
+    fill_u32(ctx, sp.stamp, V + 1, 0);
+    sp.stamp_id = &stamp_id;
+    sp.push = &push;
+  }
+  cc_core(g, method, label, sp, st);
 }
 
 }  // namespace mgg

@@ -71,10 +71,17 @@ struct Buf {
 // scan buffers of the edge-balanced push operator (advance.hpp)
 struct PushWork {
   Buf deg, scan;
+  // borrowed buffers of a caller that cannot allocate (used when set), and
+  // the scan's temporary storage (null: the context's, grown on demand)
+  uint64_t *deg_ext = nullptr, *scan_ext = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
   void ensure(uint64_t V) {
     deg.ensure((V + 2) * 8);
     scan.ensure((V + 2) * 8);
   }
+  uint64_t *deg_ptr() const { return deg_ext ? deg_ext : deg.as<uint64_t>(); }
+  uint64_t *scan_ptr() const { return scan_ext ? scan_ext : scan.as<uint64_t>(); }
 };
 
 struct Context {

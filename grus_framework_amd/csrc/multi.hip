@@ -663,7 +663,7 @@ std::vector<int> MsEngine::step() {
       GRUS_LAUNCH_CHECK();
     } else if (qn_) {
       push_scan_degrees(ctx, push_, out.off, queue_.as<uint32_t>(), (uint32_t)qn_);
-      const uint64_t total = read_u64(push_.scan.as<uint64_t>() + qn_);
+      const uint64_t total = read_u64(push_.scan_ptr() + qn_);
       ms::push_job_t job{front_.as<uint64_t>(), visit_.as<uint64_t>(), next_.as<uint64_t>()};
       push_advance(ctx, push_, out, queue_.as<uint32_t>(), (uint32_t)qn_, total, job,
                    nullptr, push_cnt_.as<StepCounters>());
@@ -678,17 +678,17 @@ std::vector<int> MsEngine::step() {
     GRUS_LAUNCH_CHECK();
   } else {
     if (qn_) {
-      Buf &cs = push_.scan;
+      uint64_t *cs = push_.scan_ptr();
       ms::k_chunk_counts<<<grid_cap(ctx, qn_ + 1, BLOCK), BLOCK, 0, ctx.stream>>>(
-          out.off, queue_.as<uint32_t>(), (uint32_t)qn_, push_.deg.as<uint64_t>());
+          out.off, queue_.as<uint32_t>(), (uint32_t)qn_, push_.deg_ptr());
       GRUS_LAUNCH_CHECK();
-      scan_excl_u64(ctx, push_.deg.as<uint64_t>(), cs.as<uint64_t>(), qn_ + 1);
+      scan_excl_u64(ctx, push_.deg_ptr(), cs, qn_ + 1);
       // no read-back of the chunk count: at most one chunk per queued vertex
       // plus one per CHUNK of its edges (front_edges_ misses the degrees of
       // sources admitted since the last step, which only sizes the grid)
       const uint64_t est = qn_ + front_edges_ / ms::CHUNK;
       ms::k_sssp_relax<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          out, queue_.as<uint32_t>(), (uint32_t)qn_, cs.as<uint64_t>(), front_.as<uint64_t>(),
+          out, queue_.as<uint32_t>(), (uint32_t)qn_, cs, front_.as<uint64_t>(),
           next_.as<uint64_t>(), vals_.as<uint32_t>());
       GRUS_LAUNCH_CHECK();
     }

@@ -36,6 +36,24 @@ void scan_excl_u64(Context &ctx, const uint64_t *in, uint64_t *out, size_t n) {
                                    ctx.stream));
 }
 
+size_t scan_excl_u64_bytes(Context &ctx, size_t n) {
+  size_t bytes = 0;
+  GRUS_HIP(rocprim::exclusive_scan(nullptr, bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr,
+                                   uint64_t(0), n, rocprim::plus<uint64_t>(), ctx.stream));
+  return bytes;
+}
+
+void scan_excl_u64_with(Context &ctx, const uint64_t *in, uint64_t *out, size_t n, void *tmp,
+                        size_t tmp_bytes) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  GRUS_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, uint64_t(0), n,
+                                   rocprim::plus<uint64_t>(), ctx.stream));
+  if (bytes > tmp_bytes) raise(GRUS_E_BUDGET, "scan temporaries exceed the given storage");
+  GRUS_HIP(rocprim::exclusive_scan(tmp, bytes, in, out, uint64_t(0), n, rocprim::plus<uint64_t>(),
+                                   ctx.stream));
+}
+
 void scan_excl_u32(Context &ctx, const uint32_t *in, uint32_t *out, size_t n) {
   if (n == 0) return;
   size_t bytes = 0;

@@ -7,6 +7,11 @@ namespace mgg {
 
 // out[i] = sum of in[0..i), for i in [0, n); in and out must not alias.
 void scan_excl_u64(Context &ctx, const uint64_t *in, uint64_t *out, size_t n);
+// The same with caller-provided temporary storage (no allocation): raises
+// GRUS_E_BUDGET when tmp_bytes is below scan_excl_u64_bytes(n).
+void scan_excl_u64_with(Context &ctx, const uint64_t *in, uint64_t *out, size_t n, void *tmp,
+                        size_t tmp_bytes);
+size_t scan_excl_u64_bytes(Context &ctx, size_t n);
 // out[i] = sum of in[0..i) in 32 bits (the caller guarantees the total fits).
 void scan_excl_u32(Context &ctx, const uint32_t *in, uint32_t *out, size_t n);
 // Exclusive scan of the 0/1 "first of its run and not the sentinel" flags of

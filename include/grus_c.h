@@ -76,8 +76,10 @@ typedef struct grus_place {
                              grus_set_device_budget does (the reference caps
                              the whole device, tools/mem_limit.cu:6-18, and
                              places from cudaMemGetInfo, include/graph.cuh:
-                             171-205).  The device build of a generated graph
-                             and its temporaries come before the cap, as the
+                             171-205).  The workspace reserve is
+                             48 (V + 2) + 16 (E / 2048 + 2) + 64 MiB bytes.
+                             The device build of a generated graph and its
+                             temporaries come before the cap, as the
                              reference's loader fills host memory first. */
 } grus_place;
 
@@ -231,6 +233,28 @@ typedef struct grus_graph_info {
   uint64_t host_bytes;        /* host-resident edge bytes (CAPPED/MANAGED) */
 } grus_graph_info;
 int grus_graph_info_get(grus_graph *g, grus_graph_info *out);
+/* Device view of a graph's edges for user operators (include/grus_ops.hpp):
+ * graph_t<CSR> / graph_t<CSC> as the reference's kernels receive them
+ * (include/graph.cuh:113-268, passed by value into kernels).  Edges
+ * [split, E) are in host memory (capped placement; split == E otherwise),
+ * as graph_chunk::access_edge splits them (include/graph.cuh:70-75).  The
+ * pointers stay valid until the graph is destroyed. */
+typedef struct grus_csr_view {
+  const uint64_t *off;     /* V + 1 row offsets */
+  const uint32_t *col;     /* edges [0, split) */
+  const uint32_t *w;       /* their weights, NULL when unweighted */
+  const uint32_t *col_far; /* edges [split, E) */
+  const uint32_t *w_far;
+  uint64_t split, V, E;
+} grus_csr_view;
+/* in_edges = 0: out-edges (CSR); 1: in-edges (CSC, built on the device on
+ * first use for a directed graph, as csr_tocsc does, include/graph.cuh:
+ * 273-303; aliases the CSR for a symmetric one) */
+int grus_graph_view(grus_graph *g, int in_edges, grus_csr_view *out);
+/* The context's HIP stream (a hipStream_t), on which every library call of
+ * the context runs: user kernels launched on it are ordered with them.
+ * grus_set_cu_mask replaces the stream; query it again afterwards. */
+int grus_ctx_stream(grus_ctx *ctx, void **hip_stream);
 /* Copy the CSR back to host buffers (any pointer may be NULL). */
 int grus_graph_copy_csr(grus_graph *g, uint64_t *off, uint32_t *col, uint32_t *w);
 int grus_graph_destroy(grus_graph *g);

@@ -75,7 +75,10 @@ def test_device_budget_caps_the_whole_footprint(ctx, reference):
     src, (off, col, w), ref = reference
     V, E = len(off) - 1, len(col)
     held0, _ = ctx.device_memory()
-    budget = held0 + (96 << 20)  # workspaces (~64 MB slack) + part of the edges
+    # the documented reserve beside the graph (grus_c.h, device_budget_bytes)
+    reserve = 48 * (V + 2) + 16 * (E // 2048 + 2) + (64 << 20)
+    fixed = (V + 1) * 8 + V * 4  # offsets and first neighbours
+    budget = held0 + reserve + fixed + (E // 2) * 8  # room for about half the edges
     try:
         pl = grus.place("capped", device_budget_bytes=budget)
         with grus.Graph.from_csr(ctx, off, col, w, symmetric=True, placement=pl) as g:
