@@ -308,12 +308,14 @@ class Server:
     """Continuous-batching query server (grus_server_*)."""
 
     def __init__(self, g, max_inflight_batches=0, max_batch=0, state_budget_bytes=0,
-                 copy_results=False, admit_min_free=0):
-        """Zero for max_inflight_batches, max_batch or admit_min_free picks the
-        library default (4 engines per kind, 64 lanes, admit into idle engines)."""
+                 copy_results=False, admit_min_free=0, admit_window=0):
+        """Zero for max_inflight_batches, max_batch, admit_min_free or
+        admit_window picks the library default (4 engines per kind, 64 lanes,
+        a batch admits arrivals during its first 2 steps, later only when
+        idle); admit_window < 0 turns the window off."""
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
-                               int(copy_results), admit_min_free)
+                               int(copy_results), admit_min_free, admit_window)
         h = C.c_void_p()
         capi.check(_lib().grus_server_create(g.h, C.byref(opts), C.byref(h)))
         self.h = h

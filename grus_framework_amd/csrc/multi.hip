@@ -69,17 +69,34 @@ __global__ void k_reset_rows(uint32_t *level, uint64_t V, uint64_t mask) {
   }
 }
 
-// SSSP columns (vertex-major [V][64]) of lanes in mask <- INFINIT
-__global__ void k_reset_cols(uint32_t *dist, uint64_t V, uint64_t mask) {
+// 16-bit SSSP rows: 0xFFFF marks an unreached entry
+constexpr uint32_t INF16 = 0xFFFFu;
+__device__ __forceinline__ uint32_t wide_dist(uint16_t x) { return x == INF16 ? INFINIT : x; }
+__device__ __forceinline__ uint32_t wide_dist(uint32_t x) { return x; }
+template <class T>
+constexpr uint32_t inf_of() {
+  return sizeof(T) == 2 ? INF16 : INFINIT;
+}
+
+// SSSP columns (vertex-major [V][64]) of lanes in mask <- unreached
+template <class T>
+__global__ void k_reset_cols(T *dist, uint64_t V, uint64_t mask) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < V * 64; i += stride)
-    if ((mask >> (i & 63)) & 1ull) dist[i] = INFINIT;
+    if ((mask >> (i & 63)) & 1ull) dist[i] = (T)inf_of<T>();
+}
+
+// 16-bit rows to 32-bit rows (0xFFFF -> INFINIT)
+__global__ void k_widen(const uint16_t *in, uint32_t *out, uint64_t n) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = wide_dist(in[i]);
 }
 
 __global__ void k_inject(const uint32_t *srcs, const uint32_t *lanes, int n, bool bfs,
-                         uint64_t *visit, uint64_t *front, uint32_t *vals, uint64_t V,
-                         uint32_t *queue, MsCounters *cnt) {
+                         bool narrow, uint64_t *visit, uint64_t *front, uint32_t *vals,
+                         uint64_t V, uint32_t *queue, MsCounters *cnt) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   uint32_t s = srcs[k], lane = lanes[k];
@@ -87,6 +104,8 @@ __global__ void k_inject(const uint32_t *srcs, const uint32_t *lanes, int n, boo
   if (bfs) {
     atomicOr((unsigned long long *)&visit[s], bit);
     vals[(uint64_t)lane * V + s] = 0;  // level[lane][s]
+  } else if (narrow) {
+    reinterpret_cast<uint16_t *>(vals)[(uint64_t)s * 64 + lane] = 0;  // dist[s][lane]
   } else {
     vals[(uint64_t)s * 64 + lane] = 0;  // dist[s][lane]
   }
@@ -321,6 +340,147 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t 
   }
 }
 
+// Lowers 16-bit half `half` of *word to nd (a previous read of the word is
+// `old`); returns when the half is at most nd.  Values only decrease, so
+// every failed compare-and-swap means another lane lowered the word and the
+// loop ends after finitely many of them.
+__device__ __forceinline__ void cas_min16(uint32_t *word, uint32_t half, uint32_t nd,
+                                          uint32_t old) {
+  for (;;) {
+    const uint32_t cur = half ? (old >> 16) : (old & 0xFFFFu);
+    if (nd >= cur) return;
+    const uint32_t nw = half ? ((old & 0xFFFFu) | (nd << 16)) : ((old & 0xFFFF0000u) | nd);
+    const uint32_t prev = atomicCAS(word, old, nw);
+    if (prev == old) return;
+    old = prev;
+  }
+}
+
+// k_sssp_relax over 16-bit rows: a vertex's 64 distances are one 128-byte
+// line, half the bytes per row visit.  The device has no 16-bit atomic
+// minimum, so each pair of queries shares a 32-bit word that the even lane
+// lowers by compare-and-swap: the row's plain reads give the expected word,
+// the CAS of all edges of an 8-edge group are issued before any result is
+// awaited, and the rare failures retry from the value returned.  A lane
+// whose new distance would be 0xFFFF or more where the row holds "unreached"
+// is flagged in cnt->overflow instead of stored.
+__global__ __launch_bounds__(BLOCK) void k_sssp_relax16(CsrView g, const uint32_t *q, uint32_t F,
+                                                        const uint64_t *CS, const uint64_t *front,
+                                                        uint64_t *next, uint16_t *dist,
+                                                        MsCounters *cnt) {
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint32_t lane = lane_id();
+  const uint64_t chunks = CS[F];
+  uint32_t *const dw = reinterpret_cast<uint32_t *>(dist);
+  uint64_t ovf = 0;  // lanes (queries) that need more than 16 bits
+  const uint64_t per = (chunks + nwaves - 1) / nwaves;
+  const uint64_t c0 = wave * per;
+  const uint64_t c1 = c0 + per < chunks ? c0 + per : chunks;
+  if (c0 < c1) {
+    uint32_t jb = upper_bound_u64(CS, 0, F + 1, c0) - 1;  // owner of c0
+    uint64_t w_cs = 0, w_off = 0, w_end = 0, w_fm = 0, wend = 0;
+    uint32_t w_u = 0;
+    auto load_window = [&]() {  // queue entries jb .. jb+63
+      const uint64_t jl = (uint64_t)jb + lane;
+      w_cs = jl <= F ? CS[jl] : ~0ull;
+      const bool in = jl < F;
+      w_u = in ? q[jl] : 0u;
+      w_off = in ? g.off[w_u] : 0ull;
+      w_end = in ? g.off[w_u + 1] : 0ull;
+      w_fm = in ? front[w_u] : 0ull;
+      wend = (uint64_t)jb + WAVE <= F ? CS[jb + WAVE] : chunks;
+    };
+    load_window();
+    for (uint64_t c = c0; c < c1; c++) {
+      while (c >= wend) {
+        jb += WAVE;
+        load_window();
+      }
+      const uint64_t below = __ballot(w_cs <= c);
+      const uint32_t l = 63u - (uint32_t)__clzll((long long)below);
+      const uint32_t u = __builtin_amdgcn_readlane(w_u, l);
+      const uint64_t e = readlane_u64(w_off, l) + (c - readlane_u64(w_cs, l)) * CHUNK;
+      uint64_t e1 = readlane_u64(w_end, l);
+      if (e1 > e + CHUNK) e1 = e + CHUNK;
+      const uint32_t n = (uint32_t)(e1 - e);  // 1..64, wave-uniform
+      const bool valid = lane < n;
+      const uint32_t my_v = valid ? edge_col(g, e + lane) : u;
+      const uint32_t my_w = valid ? edge_w(g, e + lane) : 0u;
+      const uint64_t fm = readlane_u64(w_fm, l);
+      const uint32_t du_l = dist[(uint64_t)u * 64 + lane];  // lane l: query l's distance at u
+      if (__popcll(fm) <= SPARSE_LANES) {
+        // few queries active at u: lanes take different edges and loop over
+        // the active query bits (wave-uniform loop)
+        uint64_t imp = 0;
+        for (uint64_t m = fm; m; m &= m - 1) {
+          const uint32_t qb = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+          const uint32_t du = __builtin_amdgcn_readlane(du_l, qb);
+          if (!valid || du == INF16) continue;
+          const uint64_t nd = (uint64_t)du + my_w;
+          const uint64_t slot = (uint64_t)my_v * 64 + qb;
+          const uint32_t wv = dw[slot >> 1];
+          const uint32_t cur = (qb & 1) ? (wv >> 16) : (wv & 0xFFFFu);
+          if (nd >= cur) {
+            if (cur == INF16) ovf |= 1ull << qb;  // would need a 17th bit
+            continue;
+          }
+          cas_min16(&dw[slot >> 1], qb & 1, (uint32_t)nd, wv);
+          imp |= 1ull << qb;
+        }
+        if (imp) atomicOr((unsigned long long *)&next[my_v], imp);
+        continue;
+      }
+      const bool act = ((fm >> lane) & 1ull) && du_l != INF16;
+      constexpr uint32_t U = 8;
+      for (uint32_t k0 = 0; k0 < n; k0 += U) {
+        uint32_t vv[U], dd[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {  // issue up to U row reads
+          const uint32_t jj = k0 + k < n ? k0 + k : n - 1;
+          vv[k] = __builtin_amdgcn_readlane(my_v, jj);
+          dd[k] = dist[(uint64_t)vv[k] * 64 + lane];
+        }
+        uint32_t old[U], want[U];
+        bool need[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+          const bool in = k0 + k < n;
+          const uint64_t nd = (uint64_t)du_l + __builtin_amdgcn_readlane(my_w, in ? k0 + k : 0);
+          const bool imp = in && act && nd < dd[k];
+          if (in && act && !imp && dd[k] == INF16) ovf |= 1ull << lane;
+          const uint64_t m = __ballot(imp);
+          if (m && lane == 0) atomicOr((unsigned long long *)&next[vv[k]], m);
+          // the even lane of each pair builds the pair's word
+          const uint32_t cand = imp ? (uint32_t)nd : INF16;
+          const uint32_t p_cand = __shfl_down(cand, 1);
+          const uint32_t p_dd = __shfl_down(dd[k], 1);
+          old[k] = (p_dd << 16) | dd[k];
+          const uint32_t lo = cand < dd[k] ? cand : dd[k];
+          const uint32_t hi = p_cand < p_dd ? p_cand : p_dd;
+          want[k] = (hi << 16) | lo;
+          need[k] = !(lane & 1) && want[k] != old[k];
+        }
+        uint32_t prev[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++)  // every CAS in flight before any result is used
+          prev[k] = need[k] ? atomicCAS(&dw[((uint64_t)vv[k] * 64 + lane) >> 1], old[k], want[k])
+                            : old[k];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+          if (!need[k] || prev[k] == old[k]) continue;
+          // another lane lowered the word first: lower each half from its value
+          uint32_t *wp = &dw[((uint64_t)vv[k] * 64 + lane) >> 1];
+          cas_min16(wp, 0, want[k] & 0xFFFFu, prev[k]);
+          cas_min16(wp, 1, want[k] >> 16, *wp);
+        }
+      }
+    }
+  }
+  const uint64_t o = wave_or(ovf);
+  if (o && lane == 0) atomicOr((unsigned long long *)&cnt->overflow, o);
+}
+
 // A workgroup owns a contiguous span of `items` x BLOCK vertices (items <= 32,
 // one bit per item in `bits`): it commits the next bits of its span, counts
 // its new frontier vertices, reserves their queue slots with one atomic and
@@ -395,10 +555,11 @@ __global__ void k_publish(uint64_t *d, uint64_t *h, uint32_t n) {
 }
 
 // ---- extraction ----
-__global__ void k_gather_col(const uint32_t *dist, uint64_t V, uint32_t lane, uint32_t *row) {
+template <class T>
+__global__ void k_gather_col(const T *dist, uint64_t V, uint32_t lane, uint32_t *row) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < V; i += stride) row[i] = dist[i * 64 + lane];
+  for (; i < V; i += stride) row[i] = wide_dist(dist[i * 64 + lane]);
 }
 
 // the block's sums land in out[0], out[1] with one atomic pair per block
@@ -460,7 +621,8 @@ __global__ __launch_bounds__(BLOCK) void k_digest_rows(const uint32_t *vals, uin
 // digests of the finished SSSP lanes in one sweep of the vertex-major
 // [V][64] state: lane l of every wave reads column l (coalesced 256-byte
 // rows) and sums its own lane's terms
-__global__ __launch_bounds__(BLOCK) void k_digest_cols(const uint32_t *vals, uint64_t V,
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_digest_cols(const T *vals, uint64_t V,
                                                        uint64_t mask, unsigned long long *out) {
   constexpr int NW = BLOCK / WAVE;
   __shared__ unsigned long long s_h[NW][WAVE], s_n[NW][WAVE];
@@ -471,7 +633,7 @@ __global__ __launch_bounds__(BLOCK) void k_digest_cols(const uint32_t *vals, uin
   unsigned long long h = 0, n = 0;
   if (mine) {
     for (uint64_t v = wave; v < V; v += nwaves) {
-      const uint32_t x = vals[v * 64 + lane];
+      const uint32_t x = wide_dist(vals[v * 64 + lane]);
       h += digest_term(v, x);
       n += (x != INFINIT);
     }
@@ -530,7 +692,8 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
     vals_.alloc((uint64_t)lanes_ * V * 4 + 8);  // level[lane][v]
     fill_u64(ctx, visit_.as<uint64_t>(), V + 1, 0);
   } else {
-    vals_.alloc(V * 64 * 4 + 8);  // dist[v][lane]
+    narrow_ = true;
+    vals_.alloc(V * 64 * 2 + 8);  // dist[v][lane], 16 bits until a lane needs more
   }
   fill_u64(ctx, front_.as<uint64_t>(), V + 1, 0);
   fill_u64(ctx, next_.as<uint64_t>(), V + 1, 0);
@@ -565,11 +728,61 @@ void MsEngine::reset_lanes(uint64_t mask) {
   if (kind_ == MsKind::BFS) {
     ms::k_reset_rows<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V,
                                                                        mask);
+  } else if (narrow_) {
+    ms::k_reset_cols<uint16_t><<<grid_cap(ctx, V * 64, BLOCK), BLOCK, 0, ctx.stream>>>(
+        vals_.as<uint16_t>(), V, mask);
   } else {
-    ms::k_reset_cols<<<grid_cap(ctx, V * 64, BLOCK), BLOCK, 0, ctx.stream>>>(
+    ms::k_reset_cols<uint32_t><<<grid_cap(ctx, V * 64, BLOCK), BLOCK, 0, ctx.stream>>>(
         vals_.as<uint32_t>(), V, mask);
   }
   GRUS_LAUNCH_CHECK();
+}
+
+// sources into their lanes and onto the frontier queue (qn_ entries so far)
+void MsEngine::inject(const uint32_t *srcs, const int *lanes, int n) {
+  Context &ctx = *ctx_;
+  uint32_t hs[128];  // srcs then lanes
+  for (int k = 0; k < n; k++) {
+    hs[k] = srcs[k];
+    hs[64 + k] = (uint32_t)lanes[k];
+    src_[lanes[k]] = srcs[k];
+  }
+  auto *cnt = cnt_.as<MsCounters>();
+  MsCounters zero{};
+  zero.qcount = (uint32_t)qn_;
+  GRUS_HIP(hipMemcpyAsync(cnt, &zero, sizeof zero, hipMemcpyHostToDevice, ctx.stream));
+  GRUS_HIP(hipMemcpyAsync(srcs_dev_.p, hs, sizeof hs, hipMemcpyHostToDevice, ctx.stream));
+  ms::k_inject<<<1, 64, 0, ctx.stream>>>(srcs_dev_.as<uint32_t>(), srcs_dev_.as<uint32_t>() + 64,
+                                         n, kind_ == MsKind::BFS, narrow_, visit_.as<uint64_t>(),
+                                         front_.as<uint64_t>(), vals_.as<uint32_t>(), g_.V,
+                                         queue_.as<uint32_t>(), cnt);
+  GRUS_LAUNCH_CHECK();
+  const MsCounters h = publish();
+  qn_ = h.qcount;
+}
+
+void MsEngine::widen(uint64_t restart) {
+  Context &ctx = *ctx_;
+  const uint64_t n = g_.V * 64;
+  Buf wide;
+  wide.alloc(n * 4 + 8);
+  ms::k_widen<<<grid_cap(ctx, n, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint16_t>(),
+                                                                 wide.as<uint32_t>(), n);
+  GRUS_LAUNCH_CHECK();
+  ctx.sync();
+  vals_ = std::move(wide);
+  narrow_ = false;
+  if (!restart) return;
+  // the flagged lanes start over from their sources in 32-bit rows
+  reset_lanes(restart);
+  uint32_t srcs[64];
+  int lanes[64], k = 0;
+  for (uint64_t m = restart; m; m &= m - 1) {
+    const int l = __builtin_ctzll(m);
+    srcs[k] = src_[l];
+    lanes[k++] = l;
+  }
+  inject(srcs, lanes, k);
 }
 
 std::vector<int> MsEngine::admit(const uint32_t *srcs, int n) {
@@ -578,7 +791,6 @@ std::vector<int> MsEngine::admit(const uint32_t *srcs, int n) {
   std::vector<int> used;
   uint64_t all = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);
   uint64_t freem = all & ~busy_;
-  uint32_t hs[128];  // srcs then lanes
   for (int k = 0; k < n && freem; k++) {
     if (srcs[k] >= g_.V) raise(GRUS_E_INVALID, "source vertex out of range");
     int lane = __builtin_ctzll(freem);
@@ -589,26 +801,13 @@ std::vector<int> MsEngine::admit(const uint32_t *srcs, int n) {
   if (!m) return used;
   uint64_t mask = 0;
   for (int k = 0; k < m; k++) {
-    hs[k] = srcs[k];
-    hs[64 + k] = (uint32_t)used[k];
     mask |= 1ull << used[k];
     start_[used[k]] = (uint32_t)t_;
   }
   reset_lanes(mask & dirty_);
   dirty_ &= ~mask;
-  // frontier queue currently holds qn_ entries; injected sources are appended
-  auto *cnt = cnt_.as<MsCounters>();
-  MsCounters zero{};
-  zero.qcount = (uint32_t)qn_;
-  GRUS_HIP(hipMemcpyAsync(cnt, &zero, sizeof zero, hipMemcpyHostToDevice, ctx.stream));
-  GRUS_HIP(hipMemcpyAsync(srcs_dev_.p, hs, sizeof hs, hipMemcpyHostToDevice, ctx.stream));
-  ms::k_inject<<<1, 64, 0, ctx.stream>>>(srcs_dev_.as<uint32_t>(), srcs_dev_.as<uint32_t>() + 64,
-                                         m, kind_ == MsKind::BFS, visit_.as<uint64_t>(),
-                                         front_.as<uint64_t>(), vals_.as<uint32_t>(), g_.V,
-                                         queue_.as<uint32_t>(), cnt);
-  GRUS_LAUNCH_CHECK();
-  const MsCounters h = publish();
-  qn_ = h.qcount;
+  // the frontier queue holds qn_ entries; injected sources are appended
+  inject(srcs, used.data(), m);
   busy_ |= mask;
   return used;
 }
@@ -687,9 +886,15 @@ std::vector<int> MsEngine::step() {
       // plus one per CHUNK of its edges (front_edges_ misses the degrees of
       // sources admitted since the last step, which only sizes the grid)
       const uint64_t est = qn_ + front_edges_ / ms::CHUNK;
-      ms::k_sssp_relax<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          out, queue_.as<uint32_t>(), (uint32_t)qn_, cs, front_.as<uint64_t>(),
-          next_.as<uint64_t>(), vals_.as<uint32_t>());
+      if (narrow_) {
+        ms::k_sssp_relax16<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+            out, queue_.as<uint32_t>(), (uint32_t)qn_, cs, front_.as<uint64_t>(),
+            next_.as<uint64_t>(), vals_.as<uint16_t>(), cnt);
+      } else {
+        ms::k_sssp_relax<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+            out, queue_.as<uint32_t>(), (uint32_t)qn_, cs, front_.as<uint64_t>(),
+            next_.as<uint64_t>(), vals_.as<uint32_t>());
+      }
       GRUS_LAUNCH_CHECK();
     }
     // spans of up to 32 x BLOCK vertices, about 8 workgroups per CU
@@ -706,8 +911,10 @@ std::vector<int> MsEngine::step() {
   t_++;
   qn_ = h.qcount;
   front_edges_ = h.front_edges;
+  const uint64_t restart = h.overflow & busy_;
+  if (h.overflow) widen(restart);  // the flagged lanes run again, in 32-bit rows
   for (int i = 0; i < lanes_; i++) {
-    if (((running >> i) & 1ull) && !((h.alive >> i) & 1ull)) {
+    if (((running >> i) & 1ull) && !((h.alive >> i) & 1ull) && !((restart >> i) & 1ull)) {
       done.push_back(i);
       finished_mask_ |= 1ull << i;
     }
@@ -725,8 +932,12 @@ void MsEngine::extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t
     row = vals_.as<uint32_t>() + (uint64_t)lane * V;
   } else {
     row = device_out ? device_out : row_.as<uint32_t>();
-    ms::k_gather_col<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V,
-                                                                       (uint32_t)lane, row);
+    if (narrow_)
+      ms::k_gather_col<uint16_t><<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+          vals_.as<uint16_t>(), V, (uint32_t)lane, row);
+    else
+      ms::k_gather_col<uint32_t><<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+          vals_.as<uint32_t>(), V, (uint32_t)lane, row);
     GRUS_LAUNCH_CHECK();
   }
   if (device_out && row != device_out)
@@ -759,8 +970,11 @@ void MsEngine::digest(uint64_t mask, uint64_t *reached, uint64_t *checksum) {
   if (kind_ == MsKind::BFS) {
     dim3 grid(grid_cap(ctx, V, BLOCK, 2), (unsigned)__builtin_popcountll(mask));
     ms::k_digest_rows<<<grid, BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V, mask, d);
+  } else if (narrow_) {
+    ms::k_digest_cols<uint16_t><<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
+        vals_.as<uint16_t>(), V, mask, d);
   } else {
-    ms::k_digest_cols<<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
+    ms::k_digest_cols<uint32_t><<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
         vals_.as<uint32_t>(), V, mask, d);
   }
   GRUS_LAUNCH_CHECK();

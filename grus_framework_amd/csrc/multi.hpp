@@ -10,6 +10,12 @@
 //   BFS  level[lane][v] = (step of discovery) + 1 - (step the lane started)
 //   SSSP dist[v][lane]  = Bellman-Ford fixed point of its own source
 // independent of which other lanes share the batch.
+// SSSP rows start 16 bits wide (64 lanes = one 128-byte line per vertex,
+// 0xFFFF = unreached).  A relaxation that would store a distance of 0xFFFF
+// or more into an unreached entry flags its lane; the engine then widens
+// every row to 32 bits and restarts the flagged lanes from their sources
+// (a Bellman-Ford fixed point does not depend on when a lane starts), so
+// every result is exact either way.
 #pragma once
 
 #include <vector>
@@ -29,7 +35,8 @@ struct MsCounters {
   uint64_t front_edges;   // out-edges of the new frontier (direction heuristic)
   uint32_t qcount;        // frontier queue length
   uint32_t long_count;    // pull: vertices handed to wave-cooperative scan
-  uint64_t pad[4];
+  uint64_t overflow;      // SSSP, 16-bit rows: lanes with a distance past 0xFFFE
+  uint64_t pad[3];
 };
 
 class MsEngine {
@@ -56,6 +63,7 @@ class MsEngine {
   Context &ctx() { return *ctx_; }
   uint64_t steps() const { return t_; }
   uint64_t state_bytes() const;
+  bool narrow() const { return narrow_; }  // SSSP rows of 16 bits
 
  private:
   Graph &g_;
@@ -67,12 +75,17 @@ class MsEngine {
   uint64_t finished_mask_ = 0;  // busy lanes whose frontier has emptied
   uint64_t dirty_ = 0;     // lanes whose bits must be cleared before reuse
   uint32_t start_[64] = {};
+  uint32_t src_[64] = {};  // each lane's source (a widened SSSP lane restarts from it)
+  bool narrow_ = false;
   uint64_t t_ = 0;         // global step counter
   uint64_t qn_ = 0, front_edges_ = 0;
   Buf visit_, front_, next_, vals_, queue_, longq_, row_, digest_, srcs_dev_, cnt_, push_cnt_;
   Buf hmirror_;  // pinned host words: the counters as published, then read-back scratch
   PushWork push_;
   void reset_lanes(uint64_t mask);
+  void inject(const uint32_t *srcs, const int *lanes, int n);
+  // 16-bit SSSP rows to 32 bits; the lanes in `restart` start over
+  void widen(uint64_t restart);
   MsCounters publish();
   uint64_t read_u64(const uint64_t *d);
 };

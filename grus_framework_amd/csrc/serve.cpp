@@ -4,24 +4,23 @@
 // The graph stays resident; each query kind (BFS, SSSP) has one or more
 // engines, each owning 1..64 lane slots of device state and its own HIP
 // stream, driven by its own worker thread.  Between two steps of its engine
-// a worker admits waiting queries into free lanes (slot + state memory
-// available) once at least admit_min_free lanes are free, so queries that
-// arrive while others run join the batch in flight, and engines of different
-// kinds overlap on the device.  A lane whose frontier empties is finished: its
-// result is digested (and copied to the caller's buffer when requested) and
-// the slot is freed for the next query.
+// a worker admits waiting queries into free lanes, so queries that arrive
+// while others run join the batch in flight, and engines of different kinds
+// overlap on the device.  A lane whose frontier empties is finished: at the
+// end of that step its result is digested (and copied to the caller's
+// buffer when requested), the query is reported, and the slot is freed.
 //
-// The default admits only into an idle engine, four engines per kind.  Lanes
-// of one batch then stay in step: on small-world graphs their frontiers cover
-// mostly the same vertices at the same step, and a step of the engine costs
-// about one sweep of their union.  Lanes admitted one at a time drift apart,
-// every step sweeps a large union, and the engine spends its bandwidth on few
-// queries at a time (RMAT scale 22, 50/50 BFS/SSSP, saturated, 4 engines per
-// kind: 2780 queries/s for idle-only admission against 1760 for any-free-lane
-// admission).  Four engines overlap the small early and late steps of their
-// batches: 2780 queries/s against 2470 with two, at a higher p99 under
-// moderate load (0.4-0.6 s against 0.2 s), when the engines' small batches
-// share the device.
+// Admission.  An idle engine takes up to a full batch.  A running engine
+// takes arrivals into its free lanes during the first admit_window steps
+// after it started from idle (the batch is still young: the newcomers trail
+// its first lanes by a step or two), and later only once at least
+// admit_min_free lanes are free.  Lanes admitted together stay in step: on
+// small-world graphs their frontiers cover mostly the same vertices at the
+// same step, and a step of the engine costs about one sweep of their union.
+// Lanes admitted one at a time drift apart, every step sweeps a large union,
+// and the engine spends its bandwidth on few queries at a time (RMAT scale
+// 22, 50/50 BFS/SSSP, saturated, 4 engines per kind: 2780 queries/s when
+// only idle engines admit against 1760 when any free lane admits).
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -58,13 +57,16 @@ class ServerImpl : public Server {
     opts_.state_budget_bytes = 0;
     opts_.copy_results = 0;
     opts_.admit_min_free = 0;
+    opts_.admit_window = 0;
     if (o) {
       if (o->max_inflight_batches > 0) opts_.max_inflight_batches = o->max_inflight_batches;
       if (o->max_batch > 0) opts_.max_batch = o->max_batch;
       opts_.state_budget_bytes = o->state_budget_bytes;
       opts_.copy_results = o->copy_results;
       opts_.admit_min_free = o->admit_min_free;
+      opts_.admit_window = o->admit_window;
     }
+    window_ = opts_.admit_window == 0 ? 2 : std::max(0, opts_.admit_window);
     if (opts_.max_batch > 64) raise(GRUS_E_INVALID, "max_batch is at most 64");
     if (opts_.admit_min_free < 0 || opts_.admit_min_free > opts_.max_batch)
       raise(GRUS_E_INVALID, "admit_min_free is between 0 and max_batch");
@@ -145,7 +147,7 @@ class ServerImpl : public Server {
     MsEngine &eng = *engines_[ei];
     const int kind = (int)eng.kind();
     std::vector<Query> lane_q(64);
-    uint64_t done_lanes = 0;  // finished lanes not yet digested and released
+    uint64_t cohort_start = 0;  // engine step at which the running batch started
     try {
       GRUS_HIP(hipSetDevice(g_.ctx->device));
       for (;;) {
@@ -155,16 +157,19 @@ class ServerImpl : public Server {
           cv_work_.wait(lk, [&] { return stop_ || !pending_[kind].empty() || eng.active(); });
           if (stop_ && !eng.active()) break;
           int room = stop_ ? 0 : eng.free_lanes();
-          // a running batch takes newcomers only once enough lanes are free:
-          // lanes admitted together keep their frontiers in step, so a step
-          // of the engine pays for one union of mostly shared vertices
-          if (eng.active() && room < min_free_) room = 0;
+          // a running batch takes newcomers while it is young, or once
+          // enough of its lanes are free: lanes that start together keep
+          // their frontiers in step, so a step of the engine pays for one
+          // union of mostly shared vertices
+          if (eng.active() && eng.steps() - cohort_start >= (uint64_t)window_ && room < min_free_)
+            room = 0;
           while (room-- > 0 && !pending_[kind].empty()) {
             take.push_back(pending_[kind].front());
             pending_[kind].pop_front();
           }
         }
         if (!take.empty()) {
+          if (!eng.active()) cohort_start = eng.steps();
           std::vector<uint32_t> srcs;
           for (auto &q : take) srcs.push_back(q.src);
           std::vector<int> lanes = eng.admit(srcs.data(), (int)srcs.size());
@@ -175,25 +180,20 @@ class ServerImpl : public Server {
           }
         }
         if (!eng.active()) continue;
-        for (int lane : eng.step()) done_lanes |= 1ull << lane;
-        // Finished lanes are digested together (one pass over the engine's
-        // state, which for SSSP is the whole [V][64] array) once no lane is
-        // running or once they are enough to admit new queries into: under
-        // whole-batch admission that is one pass per batch instead of one
-        // per step in which a lane finished, and no result waits longer
-        // than its lanes could be reused.
-        const int ndone = __builtin_popcountll(done_lanes);
-        if (!done_lanes || (eng.busy_mask() != done_lanes && ndone < min_free_)) continue;
+        uint64_t done = 0;
+        for (int lane : eng.step()) done |= 1ull << lane;
+        if (!done) continue;
+        // the lanes that finished in this step are digested together (one
+        // pass over the engine's state) and reported at once
         uint64_t reached[64], checksum[64];
-        eng.digest(done_lanes, reached, checksum);
-        for (uint64_t m = done_lanes; m; m &= m - 1) {
+        eng.digest(done, reached, checksum);
+        for (uint64_t m = done; m; m &= m - 1) {
           const int lane = __builtin_ctzll(m);
           const Query &q = lane_q[lane];
           if (opts_.copy_results) eng.extract(lane, q.out, nullptr, nullptr);
           eng.release(lane);
           finish(q, GRUS_OK, reached[lane], checksum[lane]);
         }
-        done_lanes = 0;
       }
     } catch (const Error &e) {
       fail(e.code, e.what());
@@ -234,6 +234,7 @@ class ServerImpl : public Server {
   std::deque<Query> pending_[2];
   std::deque<grus_query_result> done_;
   int min_free_ = 1;
+  int window_ = 2;  // steps after an idle start during which a batch admits
   bool stop_ = false, failed_ = false;
   int failed_code_ = GRUS_OK;
   std::string failed_msg_;

@@ -298,7 +298,8 @@ int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
 /* ---- serving (no reference counterpart) ----
  * A server keeps the graph and per-query state on the device, admits
  * queries as lane slots and state memory free up, batches up to 64 queries
- * of one kind per traversal, and keeps admitting while batches run. */
+ * of one kind per traversal, and keeps admitting while batches run.  A
+ * query is reported at the end of the step in which its lane finishes. */
 #define GRUS_Q_BFS 0
 #define GRUS_Q_SSSP 1
 typedef struct grus_server_opts {
@@ -312,15 +313,20 @@ typedef struct grus_server_opts {
   uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
   int copy_results;           /* 1: copy each result to the host buffer given
                                  at submit; 0: keep a digest only */
-  int admit_min_free;         /* an engine that still has running queries
-                                 admits waiting ones only once at least this
-                                 many of its lanes are free (an idle engine
-                                 always admits).  1 = admit into any free lane
-                                 at once (fully continuous); larger values keep
-                                 a batch's queries in step, which shrinks the
-                                 union of their frontiers.  0 = default:
-                                 max_batch (admit into an idle engine only);
-                                 at most max_batch */
+  int admit_min_free;         /* past its admission window, an engine that
+                                 still has running queries admits waiting ones
+                                 only once at least this many of its lanes are
+                                 free (an idle engine always admits).  1 =
+                                 admit into any free lane at once (fully
+                                 continuous); larger values keep a batch's
+                                 queries in step, which shrinks the union of
+                                 their frontiers.  0 = default: max_batch; at
+                                 most max_batch */
+  int admit_window;           /* a batch admits arrivals into its free lanes
+                                 during its first admit_window steps after the
+                                 engine started from idle, whatever
+                                 admit_min_free says (0 = default 2; negative =
+                                 no window) */
 } grus_server_opts;
 typedef struct grus_query_result {
   uint64_t id;

@@ -93,6 +93,39 @@ def test_sssp_multi_grid(ctx, oracle):
             assert (out[i] == oracle.sssp(h, s)).all()
 
 
+def test_sssp_multi_distances_past_16_bits(ctx, oracle):
+    # a heavy path (distances up to 299,000) beside a light random component:
+    # rows start 16 bits wide, the path's lanes overflow, every row widens to
+    # 32 bits and those lanes restart; every lane must still equal Dijkstra,
+    # in this batch and in the next one on the widened engine
+    rng = np.random.default_rng(21)
+    P, Q = 300, 2000
+    edges = [(i, i + 1, 1000) for i in range(P - 1)]
+    for _ in range(8000):
+        a, b = rng.integers(P, P + Q, 2)
+        if a != b:
+            edges.append((int(a), int(b), int(rng.integers(1, 11))))
+    d = {}
+    for a, b, w in edges:
+        for x, y in ((a, b), (b, a)):
+            d[(x, y)] = min(w, d.get((x, y), w))
+    keys = sorted(d)
+    V = P + Q
+    off = np.zeros(V + 1, np.uint64)
+    for a, _ in keys:
+        off[a + 1] += 1
+    off = np.cumsum(off).astype(np.uint64)
+    col = np.array([b for _, b in keys], np.uint32)
+    w = np.array([d[k] for k in keys], np.uint32)
+    h = oracle.CSR(off, col, w)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=True) as g:
+        for srcs in ([P + 5, 0, P + 77, 150, P + 1999, 299], [P + 3, 1, P + 600]):
+            out, _ = grus.sssp_multi(g, srcs)
+            for i, s in enumerate(srcs):
+                assert (out[i] == oracle.sssp(h, s)).all(), (i, s)
+        assert int(oracle.sssp(h, 0)[P - 1]) == 299000
+
+
 # ------------------------------------------------------------------ serving
 def single_digests(g, kind, srcs):
     d = {}
