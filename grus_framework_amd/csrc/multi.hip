@@ -14,6 +14,7 @@
 // chunk of up to 64 edges of one frontier vertex.
 #include <algorithm>
 #include <cstring>
+#include <string>
 
 #include "advance.hpp"
 #include "device.hpp"
@@ -26,7 +27,11 @@ namespace ms {
 
 constexpr uint32_t K_PULL = 32;
 constexpr uint32_t CHUNK = 64;       // SSSP relax: out-edges per wave task (one per lane)
-constexpr int SPARSE_LANES = 16;     // at most this many active queries: edge-parallel relax (measured best of 4..64)
+constexpr int SPARSE_LANES = 16;
+#ifndef RELAX_U16_EDGES
+#define RELAX_U16_EDGES 6
+#endif
+constexpr uint32_t RELAX_U16 = RELAX_U16_EDGES;  // 16-bit relax: edges whose CAS are in flight together     // at most this many active queries: edge-parallel relax (measured best of 4..64)
 static_assert(CHUNK == WAVE, "the edge-parallel relax covers a chunk with one edge per lane");
 
 struct LaneStart {
@@ -69,7 +74,7 @@ __global__ void k_reset_rows(uint32_t *level, uint64_t V, uint64_t mask) {
   }
 }
 
-// 16-bit SSSP rows: 0xFFFF marks an unreached entry
+// SSSP rows: 128 entries per vertex, 16 bits (0xFFFF unreached) or 32 bits
 constexpr uint32_t INF16 = 0xFFFFu;
 __device__ __forceinline__ uint32_t wide_dist(uint16_t x) { return x == INF16 ? INFINIT : x; }
 __device__ __forceinline__ uint32_t wide_dist(uint32_t x) { return x; }
@@ -77,37 +82,42 @@ template <class T>
 constexpr uint32_t inf_of() {
   return sizeof(T) == 2 ? INF16 : INFINIT;
 }
-
-// SSSP columns (vertex-major [V][64]) of lanes in mask <- unreached
-template <class T>
-__global__ void k_reset_cols(T *dist, uint64_t V, uint64_t mask) {
-  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < V * 64; i += stride)
-    if ((mask >> (i & 63)) & 1ull) dist[i] = (T)inf_of<T>();
+__device__ __forceinline__ bool slot_in(uint64_t lo, uint64_t hi, uint32_t s) {
+  return s < 64 ? ((lo >> s) & 1ull) : ((hi >> (s - 64)) & 1ull);
 }
 
-// 16-bit rows to 32-bit rows (0xFFFF -> INFINIT)
+// SSSP entries of the slots in (lo, hi) <- unreached
+template <class T>
+__global__ void k_reset_slots(T *dist, uint64_t V, uint64_t lo, uint64_t hi) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < V * 128; i += stride)
+    if (slot_in(lo, hi, (uint32_t)(i & 127))) dist[i] = (T)inf_of<T>();
+}
+
+// 16-bit entries to 32-bit entries (0xFFFF -> INFINIT)
 __global__ void k_widen(const uint16_t *in, uint32_t *out, uint64_t n) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) out[i] = wide_dist(in[i]);
 }
 
-__global__ void k_inject(const uint32_t *srcs, const uint32_t *lanes, int n, bool bfs,
+// sources into their slots: BFS slot = lane, SSSP slot s in lane s / 2
+__global__ void k_inject(const uint32_t *srcs, const uint32_t *slots, int n, bool bfs,
                          bool narrow, uint64_t *visit, uint64_t *front, uint32_t *vals,
                          uint64_t V, uint32_t *queue, MsCounters *cnt) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  uint32_t s = srcs[k], lane = lanes[k];
-  unsigned long long bit = 1ull << lane;
+  const uint32_t s = srcs[k], slot = slots[k];
+  const uint32_t lane = bfs ? slot : slot >> 1;
+  const unsigned long long bit = 1ull << lane;
   if (bfs) {
     atomicOr((unsigned long long *)&visit[s], bit);
     vals[(uint64_t)lane * V + s] = 0;  // level[lane][s]
   } else if (narrow) {
-    reinterpret_cast<uint16_t *>(vals)[(uint64_t)s * 64 + lane] = 0;  // dist[s][lane]
+    reinterpret_cast<uint16_t *>(vals)[(uint64_t)s * 128 + slot] = 0;  // dist[s][slot]
   } else {
-    vals[(uint64_t)s * 64 + lane] = 0;  // dist[s][lane]
+    vals[(uint64_t)s * 128 + slot] = 0;
   }
   unsigned long long old = atomicOr((unsigned long long *)&front[s], bit);
   if (old == 0) queue[atomicAdd(&cnt->qcount, 1u)] = s;  // not yet queued
@@ -243,105 +253,8 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t x, uint32_t i) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Relaxation of one 64-edge chunk of frontier vertex u per wave task.  Lane l
-// loads edge e+l of the chunk (coalesced), and the edges are then broadcast to
-// the wave with readlane.  Distances only decrease within a round, so a plain
-// read of dist[v] that already shows nd below it proves that dist[v] drops
-// this round: the atomics are issued without waiting for their old values, and
-// several edges' row reads are in flight before the first atomic.
-// The chunk count is read from the scan (CS[F]); the grid, sized by the
-// host from an estimate, only sets how many waves share the chunks.
-__global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t *q, uint32_t F,
-                                                      const uint64_t *CS, const uint64_t *front,
-                                                      uint64_t *next, uint32_t *dist) {
-  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
-  const uint32_t lane = lane_id();
-  const uint64_t chunks = CS[F];
-  // each wave takes a contiguous run of chunks (equal counts, so equal work
-  // within one chunk).  The queue entries its chunks belong to are read 64 at
-  // a time, one per lane (scan value, vertex, row bounds, front word): the
-  // owner of chunk c is then the highest lane whose scan value is <= c (a
-  // ballot), and the entry's fields are broadcast with readlane, so the
-  // per-chunk chain of dependent loads is gone
-  const uint64_t per = (chunks + nwaves - 1) / nwaves;
-  const uint64_t c0 = wave * per;
-  const uint64_t c1 = c0 + per < chunks ? c0 + per : chunks;
-  if (c0 >= c1) return;
-  uint32_t jb = upper_bound_u64(CS, 0, F + 1, c0) - 1;  // owner of c0
-  uint64_t w_cs = 0, w_off = 0, w_end = 0, w_fm = 0, wend = 0;
-  uint32_t w_u = 0;
-  auto load_window = [&]() {  // queue entries jb .. jb+63
-    const uint64_t jl = (uint64_t)jb + lane;
-    w_cs = jl <= F ? CS[jl] : ~0ull;  // CS[F] = chunks: never an owner
-    const bool in = jl < F;
-    w_u = in ? q[jl] : 0u;
-    w_off = in ? g.off[w_u] : 0ull;
-    w_end = in ? g.off[w_u + 1] : 0ull;
-    w_fm = in ? front[w_u] : 0ull;
-    wend = (uint64_t)jb + WAVE <= F ? CS[jb + WAVE] : chunks;  // first chunk past the window
-  };
-  load_window();
-  for (uint64_t c = c0; c < c1; c++) {
-    while (c >= wend) {  // (queued vertices without out-edges own no chunk)
-      jb += WAVE;
-      load_window();
-    }
-    const uint64_t below = __ballot(w_cs <= c);  // never 0: lane 0's scan value is <= c
-    const uint32_t l = 63u - (uint32_t)__clzll((long long)below);
-    const uint32_t u = __builtin_amdgcn_readlane(w_u, l);
-    const uint64_t e = readlane_u64(w_off, l) + (c - readlane_u64(w_cs, l)) * CHUNK;
-    uint64_t e1 = readlane_u64(w_end, l);
-    if (e1 > e + CHUNK) e1 = e + CHUNK;
-    const uint32_t n = (uint32_t)(e1 - e);  // 1..64, wave-uniform
-    const bool valid = lane < n;
-    const uint32_t my_v = valid ? edge_col(g, e + lane) : u;
-    const uint32_t my_w = valid ? edge_w(g, e + lane) : 0u;
-    const uint64_t fm = readlane_u64(w_fm, l);
-    const uint32_t du_l = dist[(uint64_t)u * 64 + lane];  // lane l: query l's distance at u
-    if (__popcll(fm) <= SPARSE_LANES) {
-      // few queries active at u: lanes take different edges and loop over
-      // the active query bits (wave-uniform loop)
-      uint64_t imp = 0;
-      for (uint64_t m = fm; m; m &= m - 1) {
-        const uint32_t qb = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-        const uint32_t du = __builtin_amdgcn_readlane(du_l, qb);
-        const uint64_t nd = (uint64_t)du + my_w;
-        uint32_t *dv = &dist[(uint64_t)my_v * 64 + qb];
-        if (valid && nd < INFINIT && nd < *dv) {
-          atomicMin(dv, (uint32_t)nd);
-          imp |= 1ull << qb;
-        }
-      }
-      if (imp) atomicOr((unsigned long long *)&next[my_v], imp);
-      continue;
-    }
-    const bool act = (fm >> lane) & 1ull;
-    const uint32_t du = act ? du_l : INFINIT;
-    constexpr uint32_t U = 8;
-    for (uint32_t k0 = 0; k0 < n; k0 += U) {
-      uint32_t vv[U], dd[U];
-#pragma unroll
-      for (uint32_t k = 0; k < U; k++) {  // issue up to U row reads
-        const uint32_t jj = k0 + k < n ? k0 + k : n - 1;
-        vv[k] = __builtin_amdgcn_readlane(my_v, jj);
-        dd[k] = dist[(uint64_t)vv[k] * 64 + lane];
-      }
-#pragma unroll
-      for (uint32_t k = 0; k < U; k++) {
-        if (k0 + k >= n) break;
-        const uint64_t nd = (uint64_t)du + __builtin_amdgcn_readlane(my_w, k0 + k);
-        const bool imp = nd < INFINIT && nd < dd[k];
-        if (imp) atomicMin(&dist[(uint64_t)vv[k] * 64 + lane], (uint32_t)nd);
-        const uint64_t m = __ballot(imp);
-        if (m && lane == 0) atomicOr((unsigned long long *)&next[vv[k]], m);
-      }
-    }
-  }
-}
-
 // Lowers 16-bit half `half` of *word to nd (a previous read of the word is
-// `old`); returns when the half is at most nd.  Values only decrease, so
+// `old`); returns once the half is at most nd.  Values only decrease, so
 // every failed compare-and-swap means another lane lowered the word and the
 // loop ends after finitely many of them.
 __device__ __forceinline__ void cas_min16(uint32_t *word, uint32_t half, uint32_t nd,
@@ -356,24 +269,51 @@ __device__ __forceinline__ void cas_min16(uint32_t *word, uint32_t half, uint32_
   }
 }
 
-// k_sssp_relax over 16-bit rows: a vertex's 64 distances are one 128-byte
-// line, half the bytes per row visit.  The device has no 16-bit atomic
-// minimum, so each pair of queries shares a 32-bit word that the even lane
-// lowers by compare-and-swap: the row's plain reads give the expected word,
-// the CAS of all edges of an 8-edge group are issued before any result is
-// awaited, and the rare failures retry from the value returned.  A lane
-// whose new distance would be 0xFFFF or more where the row holds "unreached"
-// is flagged in cnt->overflow instead of stored.
-__global__ __launch_bounds__(BLOCK) void k_sssp_relax16(CsrView g, const uint32_t *q, uint32_t F,
-                                                        const uint64_t *CS, const uint64_t *front,
-                                                        uint64_t *next, uint16_t *dist,
-                                                        MsCounters *cnt) {
+// the two distances of lane p at vertex v
+template <bool NARROW>
+__device__ __forceinline__ uint2 ld_pair(const uint32_t *dist, uint64_t v, uint32_t p) {
+  if constexpr (NARROW) {
+    const uint32_t w = dist[v * 64 + p];
+    return make_uint2(w & 0xFFFFu, w >> 16);
+  } else {
+    return reinterpret_cast<const uint2 *>(dist)[v * 64 + p];
+  }
+}
+
+// Relaxation of one 64-edge chunk of frontier vertex u per wave task, for
+// 128 queries in 64 lanes of two.  Lane l loads edge e+l of the chunk
+// (coalesced), and the edges are then broadcast to the wave with readlane.
+// Distances only decrease within a round, so a plain read of the pair at v
+// that already shows a new distance below it proves that it drops this
+// round: the next-frontier bit and the slot's "improved" bit are set from
+// that read.
+//   32-bit entries: atomicMin per improved slot, issued without waiting for
+//   the old values; the row reads of 8 edges are in flight before any.
+//   16-bit entries: the lane lowers its 32-bit word by compare-and-swap
+//   (the device has no 16-bit atomic minimum), the CAS of 8 edges issued
+//   before any result is awaited; the rare failures retry from the value
+//   returned.  A slot whose new distance would be 0xFFFF or more where it
+//   has none is flagged in cnt->overflow instead of stored.
+// The chunk count is read from the scan (CS[F]); the grid, sized by the
+// host from an estimate, only sets how many waves share the chunks.
+template <bool NARROW>
+__global__ __launch_bounds__(BLOCK) void k_sssp_relax(CsrView g, const uint32_t *q, uint32_t F,
+                                                      const uint64_t *CS, const uint64_t *front,
+                                                      uint64_t *next, uint32_t *dist,
+                                                      MsCounters *cnt) {
+  constexpr uint32_t INF = NARROW ? INF16 : INFINIT;
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
   const uint32_t lane = lane_id();
   const uint64_t chunks = CS[F];
-  uint32_t *const dw = reinterpret_cast<uint32_t *>(dist);
-  uint64_t ovf = 0;  // lanes (queries) that need more than 16 bits
+  // slots that improved a distance / that overflowed: bit p = lane p's slot
+  uint64_t imp_a = 0, imp_b = 0, ovf_a = 0, ovf_b = 0;
+  bool dimp_a = false, dimp_b = false, dovf_a = false, dovf_b = false;
+  // each wave takes a contiguous run of chunks (equal counts, so equal work
+  // within one chunk).  The queue entries its chunks belong to are read 64
+  // at a time, one per lane (scan value, vertex, row bounds, front word):
+  // the owner of chunk c is then the highest lane whose scan value is <= c
+  // (a ballot), and the entry's fields are broadcast with readlane
   const uint64_t per = (chunks + nwaves - 1) / nwaves;
   const uint64_t c0 = wave * per;
   const uint64_t c1 = c0 + per < chunks ? c0 + per : chunks;
@@ -383,21 +323,21 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_relax16(CsrView g, const uint32_
     uint32_t w_u = 0;
     auto load_window = [&]() {  // queue entries jb .. jb+63
       const uint64_t jl = (uint64_t)jb + lane;
-      w_cs = jl <= F ? CS[jl] : ~0ull;
+      w_cs = jl <= F ? CS[jl] : ~0ull;  // CS[F] = chunks: never an owner
       const bool in = jl < F;
       w_u = in ? q[jl] : 0u;
       w_off = in ? g.off[w_u] : 0ull;
       w_end = in ? g.off[w_u + 1] : 0ull;
       w_fm = in ? front[w_u] : 0ull;
-      wend = (uint64_t)jb + WAVE <= F ? CS[jb + WAVE] : chunks;
+      wend = (uint64_t)jb + WAVE <= F ? CS[jb + WAVE] : chunks;  // first chunk past the window
     };
     load_window();
     for (uint64_t c = c0; c < c1; c++) {
-      while (c >= wend) {
+      while (c >= wend) {  // (queued vertices without out-edges own no chunk)
         jb += WAVE;
         load_window();
       }
-      const uint64_t below = __ballot(w_cs <= c);
+      const uint64_t below = __ballot(w_cs <= c);  // never 0: lane 0's scan value is <= c
       const uint32_t l = 63u - (uint32_t)__clzll((long long)below);
       const uint32_t u = __builtin_amdgcn_readlane(w_u, l);
       const uint64_t e = readlane_u64(w_off, l) + (c - readlane_u64(w_cs, l)) * CHUNK;
@@ -408,77 +348,126 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_relax16(CsrView g, const uint32_
       const uint32_t my_v = valid ? edge_col(g, e + lane) : u;
       const uint32_t my_w = valid ? edge_w(g, e + lane) : 0u;
       const uint64_t fm = readlane_u64(w_fm, l);
-      const uint32_t du_l = dist[(uint64_t)u * 64 + lane];  // lane l: query l's distance at u
+      const uint2 du_l = ld_pair<NARROW>(dist, u, lane);  // lane l: its two queries' distances at u
       if (__popcll(fm) <= SPARSE_LANES) {
-        // few queries active at u: lanes take different edges and loop over
-        // the active query bits (wave-uniform loop)
-        uint64_t imp = 0;
+        // few lanes active at u: lanes take different edges and loop over
+        // the active lane bits (wave-uniform loop)
+        uint64_t nxt = 0;
         for (uint64_t m = fm; m; m &= m - 1) {
           const uint32_t qb = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-          const uint32_t du = __builtin_amdgcn_readlane(du_l, qb);
-          if (!valid || du == INF16) continue;
-          const uint64_t nd = (uint64_t)du + my_w;
-          const uint64_t slot = (uint64_t)my_v * 64 + qb;
-          const uint32_t wv = dw[slot >> 1];
-          const uint32_t cur = (qb & 1) ? (wv >> 16) : (wv & 0xFFFFu);
-          if (nd >= cur) {
-            if (cur == INF16) ovf |= 1ull << qb;  // would need a 17th bit
-            continue;
+          const uint32_t da = __builtin_amdgcn_readlane(du_l.x, qb);
+          const uint32_t db = __builtin_amdgcn_readlane(du_l.y, qb);
+          if (!valid) continue;
+          const uint64_t na = (uint64_t)da + my_w, nb = (uint64_t)db + my_w;
+          const uint2 cur = ld_pair<NARROW>(dist, my_v, qb);
+          const bool ia = da != INF && na < cur.x, ib = db != INF && nb < cur.y;
+          if (NARROW) {
+            if (da != INF && !ia && cur.x == INF) ovf_a |= 1ull << qb;
+            if (db != INF && !ib && cur.y == INF) ovf_b |= 1ull << qb;
           }
-          cas_min16(&dw[slot >> 1], qb & 1, (uint32_t)nd, wv);
-          imp |= 1ull << qb;
+          if (!ia && !ib) continue;
+          if (ia) imp_a |= 1ull << qb;
+          if (ib) imp_b |= 1ull << qb;
+          nxt |= 1ull << qb;
+          if constexpr (NARROW) {
+            uint32_t *wp = &dist[(uint64_t)my_v * 64 + qb];
+            const uint32_t old = (cur.y << 16) | cur.x;
+            if (ia) cas_min16(wp, 0, (uint32_t)na, old);
+            if (ib) cas_min16(wp, 1, (uint32_t)nb, *wp);
+          } else {
+            uint32_t *pp = &dist[((uint64_t)my_v * 64 + qb) * 2];
+            if (ia) atomicMin(pp, (uint32_t)na);
+            if (ib) atomicMin(pp + 1, (uint32_t)nb);
+          }
         }
-        if (imp) atomicOr((unsigned long long *)&next[my_v], imp);
+        if (nxt) atomicOr((unsigned long long *)&next[my_v], nxt);
         continue;
       }
-      const bool act = ((fm >> lane) & 1ull) && du_l != INF16;
-      constexpr uint32_t U = 8;
+      const bool act = (fm >> lane) & 1ull;
+      const bool act_a = act && du_l.x != INF, act_b = act && du_l.y != INF;
+      constexpr uint32_t U = NARROW ? RELAX_U16 : 8;
       for (uint32_t k0 = 0; k0 < n; k0 += U) {
-        uint32_t vv[U], dd[U];
+        uint32_t vv[U];
+        if constexpr (NARROW) {
+          // the lane's 32-bit word per edge: read, lowered value, CAS result
+          uint32_t wd[U], want[U];
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) {  // issue up to U row reads
-          const uint32_t jj = k0 + k < n ? k0 + k : n - 1;
-          vv[k] = __builtin_amdgcn_readlane(my_v, jj);
-          dd[k] = dist[(uint64_t)vv[k] * 64 + lane];
-        }
-        uint32_t old[U], want[U];
-        bool need[U];
+          for (uint32_t k = 0; k < U; k++) {  // issue up to U row reads
+            const uint32_t jj = k0 + k < n ? k0 + k : n - 1;
+            vv[k] = __builtin_amdgcn_readlane(my_v, jj);
+            wd[k] = dist[(uint64_t)vv[k] * 64 + lane];
+          }
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) {
-          const bool in = k0 + k < n;
-          const uint64_t nd = (uint64_t)du_l + __builtin_amdgcn_readlane(my_w, in ? k0 + k : 0);
-          const bool imp = in && act && nd < dd[k];
-          if (in && act && !imp && dd[k] == INF16) ovf |= 1ull << lane;
-          const uint64_t m = __ballot(imp);
-          if (m && lane == 0) atomicOr((unsigned long long *)&next[vv[k]], m);
-          // the even lane of each pair builds the pair's word
-          const uint32_t cand = imp ? (uint32_t)nd : INF16;
-          const uint32_t p_cand = __shfl_down(cand, 1);
-          const uint32_t p_dd = __shfl_down(dd[k], 1);
-          old[k] = (p_dd << 16) | dd[k];
-          const uint32_t lo = cand < dd[k] ? cand : dd[k];
-          const uint32_t hi = p_cand < p_dd ? p_cand : p_dd;
-          want[k] = (hi << 16) | lo;
-          need[k] = !(lane & 1) && want[k] != old[k];
-        }
-        uint32_t prev[U];
+          for (uint32_t k = 0; k < U; k++) {
+            const bool in = k0 + k < n;
+            // weights clamped to 2^16: any larger one gives no 16-bit distance,
+            // and the sums stay within 32 bits
+            const uint32_t w = min(__builtin_amdgcn_readlane(my_w, in ? k0 + k : 0), 0x10000u);
+            const uint32_t ca = wd[k] & 0xFFFFu, cb = wd[k] >> 16;
+            const uint32_t na = du_l.x + w, nb = du_l.y + w;
+            const bool ia = in && act_a && na < ca, ib = in && act_b && nb < cb;
+            dovf_a |= in && act_a && !ia && ca == INF;
+            dovf_b |= in && act_b && !ib && cb == INF;
+            dimp_a |= ia;
+            dimp_b |= ib;
+            const uint64_t m = __ballot(ia || ib);
+            if (m && lane == 0) atomicOr((unsigned long long *)&next[vv[k]], m);
+            want[k] = ((ib ? nb : cb) << 16) | (ia ? na : ca);
+          }
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++)  // every CAS in flight before any result is used
-          prev[k] = need[k] ? atomicCAS(&dw[((uint64_t)vv[k] * 64 + lane) >> 1], old[k], want[k])
-                            : old[k];
+          for (uint32_t k = 0; k < U; k++)  // every CAS in flight before any result is used
+            if (want[k] != wd[k])
+              want[k] = atomicCAS(&dist[(uint64_t)vv[k] * 64 + lane], wd[k], want[k]) == wd[k]
+                            ? wd[k]
+                            : want[k];
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) {
-          if (!need[k] || prev[k] == old[k]) continue;
-          // another lane lowered the word first: lower each half from its value
-          uint32_t *wp = &dw[((uint64_t)vv[k] * 64 + lane) >> 1];
-          cas_min16(wp, 0, want[k] & 0xFFFFu, prev[k]);
-          cas_min16(wp, 1, want[k] >> 16, *wp);
+          for (uint32_t k = 0; k < U; k++) {
+            if (want[k] == wd[k]) continue;  // stored, or nothing to store
+            // another wave lowered the word first: lower each half from its value
+            uint32_t *wp = &dist[(uint64_t)vv[k] * 64 + lane];
+            cas_min16(wp, 0, want[k] & 0xFFFFu, *wp);
+            cas_min16(wp, 1, want[k] >> 16, *wp);
+          }
+        } else {
+          uint2 dd[U];
+#pragma unroll
+          for (uint32_t k = 0; k < U; k++) {  // issue up to U row reads
+            const uint32_t jj = k0 + k < n ? k0 + k : n - 1;
+            vv[k] = __builtin_amdgcn_readlane(my_v, jj);
+            dd[k] = ld_pair<false>(dist, vv[k], lane);
+          }
+#pragma unroll
+          for (uint32_t k = 0; k < U; k++) {
+            const bool in = k0 + k < n;
+            const uint32_t w = __builtin_amdgcn_readlane(my_w, in ? k0 + k : 0);
+            const uint64_t na = (uint64_t)du_l.x + w, nb = (uint64_t)du_l.y + w;
+            const bool ia = in && act_a && na < dd[k].x, ib = in && act_b && nb < dd[k].y;
+            dimp_a |= ia;
+            dimp_b |= ib;
+            const uint64_t m = __ballot(ia || ib);
+            if (m && lane == 0) atomicOr((unsigned long long *)&next[vv[k]], m);
+            uint32_t *pp = &dist[((uint64_t)vv[k] * 64 + lane) * 2];
+            if (ia) atomicMin(pp, (uint32_t)na);
+            if (ib) atomicMin(pp + 1, (uint32_t)nb);
+          }
         }
       }
     }
   }
-  const uint64_t o = wave_or(ovf);
-  if (o && lane == 0) atomicOr((unsigned long long *)&cnt->overflow, o);
+  imp_a = wave_or(imp_a) | __ballot(dimp_a);
+  imp_b = wave_or(imp_b) | __ballot(dimp_b);
+  if (lane == 0) {
+    if (imp_a) atomicOr((unsigned long long *)&cnt->slot_alive[0], imp_a);
+    if (imp_b) atomicOr((unsigned long long *)&cnt->slot_alive[1], imp_b);
+  }
+  if (NARROW) {
+    ovf_a = wave_or(ovf_a) | __ballot(dovf_a);
+    ovf_b = wave_or(ovf_b) | __ballot(dovf_b);
+    if (lane == 0) {
+      if (ovf_a) atomicOr((unsigned long long *)&cnt->overflow[0], ovf_a);
+      if (ovf_b) atomicOr((unsigned long long *)&cnt->overflow[1], ovf_b);
+    }
+  }
 }
 
 // A workgroup owns a contiguous span of `items` x BLOCK vertices (items <= 32,
@@ -556,10 +545,10 @@ __global__ void k_publish(uint64_t *d, uint64_t *h, uint32_t n) {
 
 // ---- extraction ----
 template <class T>
-__global__ void k_gather_col(const T *dist, uint64_t V, uint32_t lane, uint32_t *row) {
+__global__ void k_gather_col(const T *dist, uint64_t V, uint32_t slot, uint32_t *row) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < V; i += stride) row[i] = wide_dist(dist[i * 64 + lane]);
+  for (; i < V; i += stride) row[i] = wide_dist(dist[i * 128 + slot]);
 }
 
 // the block's sums land in out[0], out[1] with one atomic pair per block
@@ -618,36 +607,49 @@ __global__ __launch_bounds__(BLOCK) void k_digest_rows(const uint32_t *vals, uin
   block_add2(h, n, out + 2 * lane);
 }
 
-// digests of the finished SSSP lanes in one sweep of the vertex-major
-// [V][64] state: lane l of every wave reads column l (coalesced 256-byte
-// rows) and sums its own lane's terms
+// digests of the finished SSSP slots in one sweep of the vertex-major
+// [V][128] state: lane p of every wave reads the entries of slots 2p and
+// 2p+1 (coalesced rows) and sums the terms of those of its slots in the mask
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_digest_cols(const T *vals, uint64_t V,
-                                                       uint64_t mask, unsigned long long *out) {
+__global__ __launch_bounds__(BLOCK) void k_digest_cols(const T *vals, uint64_t V, uint64_t lo,
+                                                       uint64_t hi, unsigned long long *out) {
   constexpr int NW = BLOCK / WAVE;
-  __shared__ unsigned long long s_h[NW][WAVE], s_n[NW][WAVE];
+  __shared__ unsigned long long s_h[NW][2 * WAVE], s_n[NW][2 * WAVE];
   const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
-  const bool mine = (mask >> lane) & 1ull;
+  const bool mine_a = slot_in(lo, hi, 2 * lane), mine_b = slot_in(lo, hi, 2 * lane + 1);
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
-  unsigned long long h = 0, n = 0;
-  if (mine) {
+  unsigned long long ha = 0, na = 0, hb = 0, nb = 0;
+  if (mine_a || mine_b) {
     for (uint64_t v = wave; v < V; v += nwaves) {
-      const uint32_t x = wide_dist(vals[v * 64 + lane]);
-      h += digest_term(v, x);
-      n += (x != INFINIT);
+      const uint32_t xa = wide_dist(vals[v * 128 + 2 * lane]);
+      const uint32_t xb = wide_dist(vals[v * 128 + 2 * lane + 1]);
+      ha += digest_term(v, xa);
+      na += (xa != INFINIT);
+      hb += digest_term(v, xb);
+      nb += (xb != INFINIT);
     }
   }
-  s_h[wid][lane] = h;
-  s_n[wid][lane] = n;
+  s_h[wid][2 * lane] = ha;
+  s_n[wid][2 * lane] = na;
+  s_h[wid][2 * lane + 1] = hb;
+  s_n[wid][2 * lane + 1] = nb;
   __syncthreads();
-  if (wid == 0 && mine) {
+  if (wid == 0) {
     for (int w = 1; w < NW; w++) {
-      h += s_h[w][lane];
-      n += s_n[w][lane];
+      ha += s_h[w][2 * lane];
+      na += s_n[w][2 * lane];
+      hb += s_h[w][2 * lane + 1];
+      nb += s_n[w][2 * lane + 1];
     }
-    atomicAdd(&out[2 * lane], h);
-    atomicAdd(&out[2 * lane + 1], n);
+    if (mine_a) {
+      atomicAdd(&out[4 * lane], ha);
+      atomicAdd(&out[4 * lane + 1], na);
+    }
+    if (mine_b) {
+      atomicAdd(&out[4 * lane + 2], hb);
+      atomicAdd(&out[4 * lane + 3], nb);
+    }
   }
 }
 
@@ -667,6 +669,7 @@ void row_digest(Context &ctx, const uint32_t *row, uint64_t V, uint64_t *reached
 // ------------------------------------------------------------------ engine
 MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
     : g_(g), kind_(kind), lanes_(std::max(1, std::min(lanes, 64))) {
+  slots_ = kind_ == MsKind::BFS ? lanes_ : 2 * lanes_;
   if (own_stream) {
     own_ctx_ = std::make_unique<Context>(g.ctx->device);
     own_ctx_->tuning = g.ctx->tuning;
@@ -680,10 +683,10 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
   if (kind_ == MsKind::BFS) visit_.alloc((V + 1) * 8);
   front_.alloc((V + 1) * 8);
   next_.alloc((V + 1) * 8);
-  queue_.alloc((V + 64) * 4);
+  queue_.alloc((V + 128) * 4);  // duplicates only from restarted slots' sources
   longq_.alloc((V + 1) * 4);
   row_.alloc((V + 1) * 4);
-  digest_.alloc(64 * 16);
+  digest_.alloc(128 * 16);
   srcs_dev_.alloc(1024);
   cnt_.alloc(sizeof(MsCounters));
   push_cnt_.alloc(sizeof(StepCounters));
@@ -693,13 +696,13 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
     fill_u64(ctx, visit_.as<uint64_t>(), V + 1, 0);
   } else {
     narrow_ = true;
-    vals_.alloc(V * 64 * 2 + 8);  // dist[v][lane], 16 bits until a lane needs more
+    vals_.alloc(V * 128 * 2 + 8);  // dist[v][slot], 16 bits until a slot needs more
   }
   fill_u64(ctx, front_.as<uint64_t>(), V + 1, 0);
   fill_u64(ctx, next_.as<uint64_t>(), V + 1, 0);
   GRUS_HIP(hipMemsetAsync(cnt_.p, 0, sizeof(MsCounters), ctx.stream));
   push_.ensure(V);
-  dirty_ = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);  // rows start uninitialised
+  dirty_ = all_slots();  // state starts uninitialised
   ctx.sync();
 }
 
@@ -712,58 +715,76 @@ uint64_t MsEngine::state_bytes() const {
          row_.bytes;
 }
 
-int MsEngine::free_lanes() const {
-  uint64_t all = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);
-  return __builtin_popcountll(all & ~busy_);
+uint64_t MsEngine::lanes_of(SlotMask m) const {
+  if (kind_ == MsKind::BFS) return (uint64_t)m;
+  uint64_t lanes = 0;
+  for (; m; m &= m - 1) lanes |= 1ull << (slot_lowest(m) >> 1);
+  return lanes;
 }
 
-void MsEngine::reset_lanes(uint64_t mask) {
+void MsEngine::reset_slots(SlotMask mask) {
   if (!mask) return;
   Context &ctx = *ctx_;
   const uint64_t V = g_.V;
-  ms::k_reset_masks<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
-      kind_ == MsKind::BFS ? visit_.as<uint64_t>() : nullptr, front_.as<uint64_t>(),
-      next_.as<uint64_t>(), V, ~mask);
-  GRUS_LAUNCH_CHECK();
   if (kind_ == MsKind::BFS) {
+    const uint64_t lanes = (uint64_t)mask;
+    ms::k_reset_masks<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+        visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), V, ~lanes);
+    GRUS_LAUNCH_CHECK();
     ms::k_reset_rows<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V,
-                                                                       mask);
-  } else if (narrow_) {
-    ms::k_reset_cols<uint16_t><<<grid_cap(ctx, V * 64, BLOCK), BLOCK, 0, ctx.stream>>>(
-        vals_.as<uint16_t>(), V, mask);
+                                                                       lanes);
+    GRUS_LAUNCH_CHECK();
+    return;
+  }
+  // a lane's frontier bits go only when neither of its slots stays busy
+  // (bits of a reset slot's lane are harmless: unreached entries relax
+  // nothing)
+  const uint64_t quiet = lanes_of(mask) & ~lanes_of(busy_ & ~mask);
+  if (quiet) {
+    ms::k_reset_masks<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
+        nullptr, front_.as<uint64_t>(), next_.as<uint64_t>(), V, ~quiet);
+    GRUS_LAUNCH_CHECK();
+  }
+  const uint64_t lo = (uint64_t)mask, hi = (uint64_t)(mask >> 64);
+  if (narrow_) {
+    ms::k_reset_slots<uint16_t><<<grid_cap(ctx, V * 128, BLOCK), BLOCK, 0, ctx.stream>>>(
+        vals_.as<uint16_t>(), V, lo, hi);
   } else {
-    ms::k_reset_cols<uint32_t><<<grid_cap(ctx, V * 64, BLOCK), BLOCK, 0, ctx.stream>>>(
-        vals_.as<uint32_t>(), V, mask);
+    ms::k_reset_slots<uint32_t><<<grid_cap(ctx, V * 128, BLOCK), BLOCK, 0, ctx.stream>>>(
+        vals_.as<uint32_t>(), V, lo, hi);
   }
   GRUS_LAUNCH_CHECK();
 }
 
-// sources into their lanes and onto the frontier queue (qn_ entries so far)
-void MsEngine::inject(const uint32_t *srcs, const int *lanes, int n) {
+// sources into their slots and onto the frontier queue (qn_ entries so far)
+void MsEngine::inject(const uint32_t *srcs, const int *slots, int n) {
   Context &ctx = *ctx_;
-  uint32_t hs[128];  // srcs then lanes
-  for (int k = 0; k < n; k++) {
-    hs[k] = srcs[k];
-    hs[64 + k] = (uint32_t)lanes[k];
-    src_[lanes[k]] = srcs[k];
-  }
   auto *cnt = cnt_.as<MsCounters>();
-  MsCounters zero{};
-  zero.qcount = (uint32_t)qn_;
-  GRUS_HIP(hipMemcpyAsync(cnt, &zero, sizeof zero, hipMemcpyHostToDevice, ctx.stream));
-  GRUS_HIP(hipMemcpyAsync(srcs_dev_.p, hs, sizeof hs, hipMemcpyHostToDevice, ctx.stream));
-  ms::k_inject<<<1, 64, 0, ctx.stream>>>(srcs_dev_.as<uint32_t>(), srcs_dev_.as<uint32_t>() + 64,
-                                         n, kind_ == MsKind::BFS, narrow_, visit_.as<uint64_t>(),
-                                         front_.as<uint64_t>(), vals_.as<uint32_t>(), g_.V,
-                                         queue_.as<uint32_t>(), cnt);
-  GRUS_LAUNCH_CHECK();
-  const MsCounters h = publish();
-  qn_ = h.qcount;
+  auto *sd = srcs_dev_.as<uint32_t>();
+  for (int k0 = 0; k0 < n; k0 += 64) {  // 64 sources per launch (srcs_dev_ holds 128 words)
+    const int m = std::min(64, n - k0);
+    uint32_t hs[128];  // sources then slots
+    for (int k = 0; k < m; k++) {
+      hs[k] = srcs[k0 + k];
+      hs[64 + k] = (uint32_t)slots[k0 + k];
+      src_[slots[k0 + k]] = srcs[k0 + k];
+    }
+    MsCounters zero{};
+    zero.qcount = (uint32_t)qn_;
+    GRUS_HIP(hipMemcpyAsync(cnt, &zero, sizeof zero, hipMemcpyHostToDevice, ctx.stream));
+    GRUS_HIP(hipMemcpyAsync(sd, hs, sizeof hs, hipMemcpyHostToDevice, ctx.stream));
+    ms::k_inject<<<1, 64, 0, ctx.stream>>>(sd, sd + 64, m, kind_ == MsKind::BFS, narrow_,
+                                           visit_.as<uint64_t>(), front_.as<uint64_t>(),
+                                           vals_.as<uint32_t>(), g_.V, queue_.as<uint32_t>(), cnt);
+    GRUS_LAUNCH_CHECK();
+    const MsCounters h = publish();
+    qn_ = h.qcount;
+  }
 }
 
-void MsEngine::widen(uint64_t restart) {
+void MsEngine::widen(SlotMask restart) {
   Context &ctx = *ctx_;
-  const uint64_t n = g_.V * 64;
+  const uint64_t n = g_.V * 128;
   Buf wide;
   wide.alloc(n * 4 + 8);
   ms::k_widen<<<grid_cap(ctx, n, BLOCK), BLOCK, 0, ctx.stream>>>(vals_.as<uint16_t>(),
@@ -773,38 +794,37 @@ void MsEngine::widen(uint64_t restart) {
   vals_ = std::move(wide);
   narrow_ = false;
   if (!restart) return;
-  // the flagged lanes start over from their sources in 32-bit rows
-  reset_lanes(restart);
-  uint32_t srcs[64];
-  int lanes[64], k = 0;
-  for (uint64_t m = restart; m; m &= m - 1) {
-    const int l = __builtin_ctzll(m);
-    srcs[k] = src_[l];
-    lanes[k++] = l;
+  // the flagged slots start over from their sources in 32-bit entries
+  reset_slots(restart);
+  uint32_t srcs[128];
+  int slots[128], k = 0;
+  for (SlotMask m = restart; m; m &= m - 1) {
+    const int sl = slot_lowest(m);
+    srcs[k] = src_[sl];
+    slots[k++] = sl;
   }
-  inject(srcs, lanes, k);
+  inject(srcs, slots, k);
 }
 
 std::vector<int> MsEngine::admit(const uint32_t *srcs, int n) {
   Context &ctx = *ctx_;
   ctx.activate();
   std::vector<int> used;
-  uint64_t all = (lanes_ == 64) ? ~0ull : ((1ull << lanes_) - 1);
-  uint64_t freem = all & ~busy_;
+  SlotMask freem = all_slots() & ~busy_;
   for (int k = 0; k < n && freem; k++) {
     if (srcs[k] >= g_.V) raise(GRUS_E_INVALID, "source vertex out of range");
-    int lane = __builtin_ctzll(freem);
+    const int sl = slot_lowest(freem);
     freem &= freem - 1;
-    used.push_back(lane);
+    used.push_back(sl);
   }
   const int m = (int)used.size();
   if (!m) return used;
-  uint64_t mask = 0;
+  SlotMask mask = 0;
   for (int k = 0; k < m; k++) {
-    mask |= 1ull << used[k];
+    mask |= slot_bit(used[k]);
     start_[used[k]] = (uint32_t)t_;
   }
-  reset_lanes(mask & dirty_);
+  reset_slots(mask & dirty_);
   dirty_ &= ~mask;
   // the frontier queue holds qn_ entries; injected sources are appended
   inject(srcs, used.data(), m);
@@ -843,21 +863,21 @@ std::vector<int> MsEngine::step() {
   const uint64_t V = g_.V;
   const CsrView out = g_.csr();
   auto *cnt = cnt_.as<MsCounters>();
-  // running lanes: busy lanes whose frontier was not empty (all busy lanes
-  // have a non-empty frontier: finished lanes are reported and leave the
-  // running set at once)
-  const uint64_t running = busy_ & ~finished_mask_;
+  // running slots: busy slots whose frontier was not empty (finished slots
+  // are reported and leave the running set at once); their lanes
+  const SlotMask running = busy_ & ~finished_;
+  const uint64_t run_lanes = lanes_of(running);
   // the counters are zero here: cleared by the previous publish()
   if (kind_ == MsKind::BFS) {
     const CsrView in = g_.csc();
     const bool pull = (double)front_edges_ > (double)g_.E / 24.0;
     if (pull) {
       ms::k_bfs_pull<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          in, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), running,
+          in, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), run_lanes,
           longq_.as<uint32_t>(), cnt);
       GRUS_LAUNCH_CHECK();
       ms::k_bfs_pull_long<<<grid_cap(ctx, V * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
-          in, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), running,
+          in, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(), run_lanes,
           longq_.as<uint32_t>(), cnt);
       GRUS_LAUNCH_CHECK();
     } else if (qn_) {
@@ -873,7 +893,7 @@ std::vector<int> MsEngine::step() {
     // which are still zero)
     ms::k_bfs_finalize<<<grid_cap(ctx, V, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
         out.off, visit_.as<uint64_t>(), front_.as<uint64_t>(), next_.as<uint64_t>(),
-        vals_.as<uint32_t>(), V, running, (uint32_t)t_, st, queue_.as<uint32_t>(), cnt);
+        vals_.as<uint32_t>(), V, run_lanes, (uint32_t)t_, st, queue_.as<uint32_t>(), cnt);
     GRUS_LAUNCH_CHECK();
   } else {
     if (qn_) {
@@ -886,15 +906,15 @@ std::vector<int> MsEngine::step() {
       // plus one per CHUNK of its edges (front_edges_ misses the degrees of
       // sources admitted since the last step, which only sizes the grid)
       const uint64_t est = qn_ + front_edges_ / ms::CHUNK;
-      if (narrow_) {
-        ms::k_sssp_relax16<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+      const unsigned grid = grid_cap(ctx, est * WAVE, BLOCK, 8);
+      if (narrow_)
+        ms::k_sssp_relax<true><<<grid, BLOCK, 0, ctx.stream>>>(
             out, queue_.as<uint32_t>(), (uint32_t)qn_, cs, front_.as<uint64_t>(),
-            next_.as<uint64_t>(), vals_.as<uint16_t>(), cnt);
-      } else {
-        ms::k_sssp_relax<<<grid_cap(ctx, est * WAVE, BLOCK, 8), BLOCK, 0, ctx.stream>>>(
+            next_.as<uint64_t>(), vals_.as<uint32_t>(), cnt);
+      else
+        ms::k_sssp_relax<false><<<grid, BLOCK, 0, ctx.stream>>>(
             out, queue_.as<uint32_t>(), (uint32_t)qn_, cs, front_.as<uint64_t>(),
-            next_.as<uint64_t>(), vals_.as<uint32_t>());
-      }
+            next_.as<uint64_t>(), vals_.as<uint32_t>(), cnt);
       GRUS_LAUNCH_CHECK();
     }
     // spans of up to 32 x BLOCK vertices, about 8 workgroups per CU
@@ -903,7 +923,7 @@ std::vector<int> MsEngine::step() {
     const uint64_t span = (uint64_t)items * BLOCK;
     const unsigned fin_grid = (unsigned)std::max<uint64_t>(1, (V + span - 1) / span);
     ms::k_sssp_finalize<<<fin_grid, BLOCK, 0, ctx.stream>>>(
-        out.off, front_.as<uint64_t>(), next_.as<uint64_t>(), V, items, running,
+        out.off, front_.as<uint64_t>(), next_.as<uint64_t>(), V, items, run_lanes,
         queue_.as<uint32_t>(), cnt);
     GRUS_LAUNCH_CHECK();
   }
@@ -911,33 +931,46 @@ std::vector<int> MsEngine::step() {
   t_++;
   qn_ = h.qcount;
   front_edges_ = h.front_edges;
-  const uint64_t restart = h.overflow & busy_;
-  if (h.overflow) widen(restart);  // the flagged lanes run again, in 32-bit rows
-  for (int i = 0; i < lanes_; i++) {
-    if (((running >> i) & 1ull) && !((h.alive >> i) & 1ull) && !((restart >> i) & 1ull)) {
-      done.push_back(i);
-      finished_mask_ |= 1ull << i;
+  SlotMask alive, restart = 0;
+  if (kind_ == MsKind::BFS) {
+    alive = (SlotMask)h.alive;
+  } else {
+    // slot 2p / 2p+1 improved a distance <=> its frontier is not empty
+    alive = 0;
+    SlotMask ovf = 0;
+    for (int p = 0; p < 64; p++) {
+      if ((h.slot_alive[0] >> p) & 1ull) alive |= slot_bit(2 * p);
+      if ((h.slot_alive[1] >> p) & 1ull) alive |= slot_bit(2 * p + 1);
+      if ((h.overflow[0] >> p) & 1ull) ovf |= slot_bit(2 * p);
+      if ((h.overflow[1] >> p) & 1ull) ovf |= slot_bit(2 * p + 1);
     }
+    restart = ovf & busy_;
+    if (ovf) widen(restart);  // the flagged slots run again, in 32-bit entries
+  }
+  for (SlotMask m = running & ~alive & ~restart; m; m &= m - 1) {
+    const int sl = slot_lowest(m);
+    done.push_back(sl);
+    finished_ |= slot_bit(sl);
   }
   return done;
 }
 
-void MsEngine::extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t *checksum,
+void MsEngine::extract(int slot, uint32_t *host_out, uint64_t *reached, uint64_t *checksum,
                        uint32_t *device_out) {
   Context &ctx = *ctx_;
   ctx.activate();
   const uint64_t V = g_.V;
   uint32_t *row;
   if (kind_ == MsKind::BFS) {
-    row = vals_.as<uint32_t>() + (uint64_t)lane * V;
+    row = vals_.as<uint32_t>() + (uint64_t)slot * V;
   } else {
     row = device_out ? device_out : row_.as<uint32_t>();
     if (narrow_)
       ms::k_gather_col<uint16_t><<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
-          vals_.as<uint16_t>(), V, (uint32_t)lane, row);
+          vals_.as<uint16_t>(), V, (uint32_t)slot, row);
     else
       ms::k_gather_col<uint32_t><<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(
-          vals_.as<uint32_t>(), V, (uint32_t)lane, row);
+          vals_.as<uint32_t>(), V, (uint32_t)slot, row);
     GRUS_LAUNCH_CHECK();
   }
   if (device_out && row != device_out)
@@ -960,38 +993,39 @@ void MsEngine::extract(int lane, uint32_t *host_out, uint64_t *reached, uint64_t
   }
 }
 
-void MsEngine::digest(uint64_t mask, uint64_t *reached, uint64_t *checksum) {
+void MsEngine::digest(SlotMask mask, uint64_t *reached, uint64_t *checksum) {
   if (!mask) return;
   Context &ctx = *ctx_;
   ctx.activate();
   const uint64_t V = g_.V;
   auto *d = digest_.as<unsigned long long>();
-  GRUS_HIP(hipMemsetAsync(d, 0, 64 * 16, ctx.stream));
+  GRUS_HIP(hipMemsetAsync(d, 0, 128 * 16, ctx.stream));
   if (kind_ == MsKind::BFS) {
-    dim3 grid(grid_cap(ctx, V, BLOCK, 2), (unsigned)__builtin_popcountll(mask));
-    ms::k_digest_rows<<<grid, BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V, mask, d);
+    const uint64_t lanes = (uint64_t)mask;
+    dim3 grid(grid_cap(ctx, V, BLOCK, 2), (unsigned)__builtin_popcountll(lanes));
+    ms::k_digest_rows<<<grid, BLOCK, 0, ctx.stream>>>(vals_.as<uint32_t>(), V, lanes, d);
   } else if (narrow_) {
     ms::k_digest_cols<uint16_t><<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
-        vals_.as<uint16_t>(), V, mask, d);
+        vals_.as<uint16_t>(), V, (uint64_t)mask, (uint64_t)(mask >> 64), d);
   } else {
     ms::k_digest_cols<uint32_t><<<grid_cap(ctx, V * WAVE, BLOCK), BLOCK, 0, ctx.stream>>>(
-        vals_.as<uint32_t>(), V, mask, d);
+        vals_.as<uint32_t>(), V, (uint64_t)mask, (uint64_t)(mask >> 64), d);
   }
   GRUS_LAUNCH_CHECK();
-  unsigned long long h[128];
+  unsigned long long h[256];
   GRUS_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx.stream));
   ctx.sync();
-  for (int l = 0; l < 64; l++)
-    if ((mask >> l) & 1ull) {
-      checksum[l] = h[2 * l];
-      reached[l] = h[2 * l + 1];
-    }
+  for (SlotMask m = mask; m; m &= m - 1) {
+    const int sl = slot_lowest(m);
+    checksum[sl] = h[2 * sl];
+    reached[sl] = h[2 * sl + 1];
+  }
 }
 
-void MsEngine::release(int lane) {
-  uint64_t bit = 1ull << lane;
+void MsEngine::release(int slot) {
+  const SlotMask bit = slot_bit(slot);
   busy_ &= ~bit;
-  finished_mask_ &= ~bit;
+  finished_ &= ~bit;
   dirty_ |= bit;
 }
 
@@ -999,25 +1033,23 @@ void MsEngine::release(int lane) {
 namespace {
 void run_multi(Graph &g, MsKind kind, const uint32_t *srcs, int k, uint32_t *out,
                grus_stats *st) {
-  if (k < 1 || k > 64) raise(GRUS_E_INVALID, "between 1 and 64 sources per batch");
+  const int max_k = kind == MsKind::BFS ? 64 : 128;
+  if (k < 1 || k > max_k)
+    raise(GRUS_E_INVALID, std::string("between 1 and ") + std::to_string(max_k) +
+                              " sources per batch");
   g.make_room(Graph::App::MULTI);
-  if (kind == MsKind::BFS) {
-    if (!g.ms || g.ms->kind() != MsKind::BFS) g.ms = std::make_shared<MsEngine>(g, kind, 64, false);
-  } else {
-    if (!g.ms || g.ms->kind() != MsKind::SSSP) g.ms = std::make_shared<MsEngine>(g, kind, 64, false);
-  }
+  if (!g.ms || g.ms->kind() != kind) g.ms = std::make_shared<MsEngine>(g, kind, 64, false);
   MsEngine &e = *g.ms;
   Context &ctx = e.ctx();
-  for (int i = 0; i < 64; i++)
-    if ((e.busy_mask() >> i) & 1ull) e.release(i);
+  for (SlotMask m = e.busy_mask(); m; m &= m - 1) e.release(slot_lowest(m));
   GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
-  std::vector<int> lanes = e.admit(srcs, k);
-  uint64_t pending = 0;
-  for (int l : lanes) pending |= 1ull << l;
+  std::vector<int> slots = e.admit(srcs, k);
+  SlotMask pending = 0;
+  for (int sl : slots) pending |= slot_bit(sl);
   uint64_t steps = 0;
   while (pending) {
-    if (++steps > g.V + 2) raise(GRUS_E_HIP, "multi-source traversal did not converge");
-    for (int l : e.step()) pending &= ~(1ull << l);
+    if (++steps > 2 * g.V + 4) raise(GRUS_E_HIP, "multi-source traversal did not converge");
+    for (int sl : e.step()) pending &= ~slot_bit(sl);
   }
   GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   GRUS_HIP(hipEventSynchronize(ctx.ev1));
@@ -1026,10 +1058,10 @@ void run_multi(Graph &g, MsKind kind, const uint32_t *srcs, int k, uint32_t *out
   uint64_t vr = 0;
   for (int i = 0; i < k; i++) {
     uint64_t r = 0;
-    e.extract(lanes[i], out ? out + (uint64_t)i * g.V : nullptr, &r, nullptr);
+    e.extract(slots[i], out ? out + (uint64_t)i * g.V : nullptr, &r, nullptr);
     vr += r;
   }
-  for (int l : lanes) e.release(l);
+  for (int sl : slots) e.release(sl);
   if (st) {
     *st = grus_stats{};
     st->device_ms = ms;

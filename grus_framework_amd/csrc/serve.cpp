@@ -10,6 +10,9 @@
 // end of that step its result is digested (and copied to the caller's
 // buffer when requested), the query is reported, and the slot is freed.
 //
+// A BFS engine holds one query per lane, an SSSP engine two (16-bit
+// distances: 128 queries per batch for the bytes of 64 in 32 bits).
+//
 // Admission.  An idle engine takes up to a full batch.  A running engine
 // takes arrivals into its free lanes during the first admit_window steps
 // after it started from idle (the batch is still young: the newcomers trail
@@ -83,9 +86,11 @@ class ServerImpl : public Server {
         raise(GRUS_E_BUDGET, "state budget too small for one lane per engine");
       lanes = (int)std::min<uint64_t>((uint64_t)lanes, (per_engine - shared) / per_lane);
     }
-    // admission gate, in lanes of an engine (capped by a budget-shrunk width)
+    // admission gate, in lanes of an engine (capped by a budget-shrunk width);
+    // an SSSP engine holds two queries per lane, so its gate in slots doubles
     min_free_ = opts_.admit_min_free ? opts_.admit_min_free : opts_.max_batch;
     min_free_ = std::min(min_free_, lanes);
+    lanes_ = lanes;
     if (!g.symmetric) g.csc();  // build the CSC once, before workers share it
     for (int k = 0; k < 2; k++)
       for (int e = 0; e < opts_.max_inflight_batches; e++)
@@ -146,7 +151,7 @@ class ServerImpl : public Server {
   void work(size_t ei) {
     MsEngine &eng = *engines_[ei];
     const int kind = (int)eng.kind();
-    std::vector<Query> lane_q(64);
+    std::vector<Query> slot_q(128);
     uint64_t cohort_start = 0;  // engine step at which the running batch started
     try {
       GRUS_HIP(hipSetDevice(g_.ctx->device));
@@ -156,12 +161,13 @@ class ServerImpl : public Server {
           std::unique_lock<std::mutex> lk(mu_);
           cv_work_.wait(lk, [&] { return stop_ || !pending_[kind].empty() || eng.active(); });
           if (stop_ && !eng.active()) break;
-          int room = stop_ ? 0 : eng.free_lanes();
+          int room = stop_ ? 0 : eng.free_slots();
           // a running batch takes newcomers while it is young, or once
           // enough of its lanes are free: lanes that start together keep
           // their frontiers in step, so a step of the engine pays for one
           // union of mostly shared vertices
-          if (eng.active() && eng.steps() - cohort_start >= (uint64_t)window_ && room < min_free_)
+          const int gate = min_free_ * (eng.slots() / lanes_);
+          if (eng.active() && eng.steps() - cohort_start >= (uint64_t)window_ && room < gate)
             room = 0;
           while (room-- > 0 && !pending_[kind].empty()) {
             take.push_back(pending_[kind].front());
@@ -172,27 +178,27 @@ class ServerImpl : public Server {
           if (!eng.active()) cohort_start = eng.steps();
           std::vector<uint32_t> srcs;
           for (auto &q : take) srcs.push_back(q.src);
-          std::vector<int> lanes = eng.admit(srcs.data(), (int)srcs.size());
+          std::vector<int> slots = eng.admit(srcs.data(), (int)srcs.size());
           double t = now_us();
-          for (size_t k = 0; k < lanes.size(); k++) {
+          for (size_t k = 0; k < slots.size(); k++) {
             take[k].start_us = t;
-            lane_q[lanes[k]] = take[k];
+            slot_q[slots[k]] = take[k];
           }
         }
         if (!eng.active()) continue;
-        uint64_t done = 0;
-        for (int lane : eng.step()) done |= 1ull << lane;
+        SlotMask done = 0;
+        for (int sl : eng.step()) done |= slot_bit(sl);
         if (!done) continue;
-        // the lanes that finished in this step are digested together (one
+        // the slots that finished in this step are digested together (one
         // pass over the engine's state) and reported at once
-        uint64_t reached[64], checksum[64];
+        uint64_t reached[128], checksum[128];
         eng.digest(done, reached, checksum);
-        for (uint64_t m = done; m; m &= m - 1) {
-          const int lane = __builtin_ctzll(m);
-          const Query &q = lane_q[lane];
-          if (opts_.copy_results) eng.extract(lane, q.out, nullptr, nullptr);
-          eng.release(lane);
-          finish(q, GRUS_OK, reached[lane], checksum[lane]);
+        for (SlotMask m = done; m; m &= m - 1) {
+          const int sl = slot_lowest(m);
+          const Query &q = slot_q[sl];
+          if (opts_.copy_results) eng.extract(sl, q.out, nullptr, nullptr);
+          eng.release(sl);
+          finish(q, GRUS_OK, reached[sl], checksum[sl]);
         }
       }
     } catch (const Error &e) {
@@ -205,8 +211,7 @@ class ServerImpl : public Server {
     int orphan_status;
     {
       std::lock_guard<std::mutex> lk(mu_);
-      for (int i = 0; i < 64; i++)
-        if ((eng.busy_mask() >> i) & 1ull) orphans.push_back(lane_q[i]);
+      for (SlotMask m = eng.busy_mask(); m; m &= m - 1) orphans.push_back(slot_q[slot_lowest(m)]);
       if (stop_) {
         for (auto &q : pending_[kind]) orphans.push_back(q);
         pending_[kind].clear();
@@ -235,6 +240,7 @@ class ServerImpl : public Server {
   std::deque<grus_query_result> done_;
   int min_free_ = 1;
   int window_ = 2;  // steps after an idle start during which a batch admits
+  int lanes_ = 64;  // lanes per engine
   bool stop_ = false, failed_ = false;
   int failed_code_ = GRUS_OK;
   std::string failed_msg_;

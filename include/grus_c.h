@@ -288,8 +288,10 @@ int grus_result_digest(grus_graph *g, uint64_t *reached, uint64_t *checksum);
 
 /* ---- batched multi-source (no reference counterpart: the north-star
  * batching pattern; each source gives exactly the single-source result) ----
- * k <= 64 sources packed as bits of one u64 per vertex.  out is k*V u32
- * (row i = result of srcs[i]); NULL keeps results on the device. */
+ * BFS: k <= 64 sources packed as bits of one u64 per vertex.  SSSP: k <= 128
+ * sources, two per bit (16-bit distances; 32-bit once a query needs it).
+ * out is k*V u32 (row i = result of srcs[i]); NULL keeps results on the
+ * device. */
 int grus_bfs_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
                    grus_stats *stats);
 int grus_sssp_multi(grus_graph *g, const uint32_t *srcs, int k, uint32_t *out,
@@ -309,7 +311,8 @@ typedef struct grus_server_opts {
                                  small steps and host round trips (more
                                  throughput at saturation; under moderate load
                                  fewer engines give lower tail latency) */
-  int max_batch;              /* lanes per batch, 1..64, default 64 */
+  int max_batch;              /* lanes per batch, 1..64, default 64; a BFS
+                                 lane holds one query, an SSSP lane two */
   uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
   int copy_results;           /* 1: copy each result to the host buffer given
                                  at submit; 0: keep a digest only */

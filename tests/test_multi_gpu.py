@@ -74,7 +74,7 @@ def test_bfs_multi_directed(ctx, oracle):
             assert (out[i] == oracle.bfs(h, s)).all()
 
 
-@pytest.mark.parametrize("k", [1, 64])
+@pytest.mark.parametrize("k", [1, 64, 128])
 def test_sssp_multi_rmat(ctx, oracle, k):
     with grus.Graph.rmat(ctx, 12, weights=(1, 255), wseed=3) as g:
         h = host_csr(g)
@@ -82,6 +82,22 @@ def test_sssp_multi_rmat(ctx, oracle, k):
         out, _ = grus.sssp_multi(g, srcs)
         for i, s in enumerate(srcs):
             assert (out[i] == oracle.sssp(h, s)).all(), (i, s)
+
+
+def test_sssp_multi_128_sources_and_errors(ctx, oracle):
+    with grus.Graph.rmat(ctx, 11, weights=(1, 255), wseed=9) as g:
+        h = host_csr(g)
+        rng = np.random.default_rng(3)
+        srcs = [int(x) for x in rng.integers(0, g.V, 128)]
+        out, _ = grus.sssp_multi(g, srcs)
+        for i in range(0, 128, 9):
+            assert (out[i] == oracle.sssp(h, srcs[i])).all(), i
+        with pytest.raises(grus.GrusError) as e:
+            grus.sssp_multi(g, list(range(129)))
+        assert e.value.code == capi.E_INVALID
+        with pytest.raises(grus.GrusError) as e:
+            grus.bfs_multi(g, list(range(65)))
+        assert e.value.code == capi.E_INVALID
 
 
 def test_sssp_multi_grid(ctx, oracle):
@@ -119,6 +135,8 @@ def test_sssp_multi_distances_past_16_bits(ctx, oracle):
     w = np.array([d[k] for k in keys], np.uint32)
     h = oracle.CSR(off, col, w)
     with grus.Graph.from_csr(ctx, off, col, w, symmetric=True) as g:
+        # slots pair up in lanes (0,1), (2,3), ...: heavy and light sources
+        # share lanes, so a lane keeps running while one of its slots restarts
         for srcs in ([P + 5, 0, P + 77, 150, P + 1999, 299], [P + 3, 1, P + 600]):
             out, _ = grus.sssp_multi(g, srcs)
             for i, s in enumerate(srcs):
