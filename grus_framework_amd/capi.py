@@ -106,6 +106,7 @@ SIGNATURES = [
     ("grus_sssp", _i, [_vp, _u32, _vp, C.POINTER(Stats)]),
     ("grus_pagerank", _i, [_vp, _f, _i, _vp, C.POINTER(Stats)]),
     ("grus_cc", _i, [_vp, _i, _vp, C.POINTER(Stats)]),
+    ("grus_cc_noalloc", _i, [_vp, _i, _vp, C.POINTER(Stats)]),
     ("grus_result_copy", _i, [_vp, _vp]),
     ("grus_result_digest", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
     ("grus_bfs_multi", _i, [_vp, _vp, _i, _vp, C.POINTER(Stats)]),

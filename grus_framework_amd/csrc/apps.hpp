@@ -18,6 +18,10 @@ void run_cc(Graph &g, int method, grus_stats *st);
 void run_cc_in(Graph &g, int method, uint32_t *labels_dev, void *scratch, uint64_t scratch_bytes,
                grus_stats *st);
 uint64_t cc_scratch_bytes(Context &ctx, uint64_t V, int method);
+// the allocation-free entry point (noalloc.hip): run_cc_in over the
+// library's per-device scratch area, set up by scratch_setup at context init
+void scratch_setup(int device);
+void run_cc_noalloc(Graph &g, int method, uint32_t *labels_dev, grus_stats *st);
 // batched multi-source (no reference counterpart); results at g.ms output
 void run_bfs_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);
 void run_sssp_multi(Graph &g, const uint32_t *srcs, int k, uint32_t *out_dev, grus_stats *st);

@@ -78,6 +78,7 @@ int grus_init(int device, grus_ctx **out) {
     *out = nullptr;
     auto c = std::make_unique<grus_ctx>();
     c->c = std::make_unique<mgg::Context>(device);  // leaves `device` current
+    mgg::scratch_setup(device);  // the scratch area of grus_cc_noalloc
     *out = c.release();
   });
 }
@@ -322,6 +323,23 @@ int grus_cc(grus_graph *g, int method, uint32_t *labels_out, grus_stats *stats) 
     mgg::run_cc(*g->g, method, stats);
     g->last_result = 1;
     copy_out(g, labels_out);
+  });
+}
+
+int grus_cc_noalloc(grus_graph *g, int method, uint32_t *labels_dev, grus_stats *stats) {
+  return guard([&] {
+    need(g, "g");
+    need(labels_dev, "labels_dev");
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, labels_dev) != hipSuccess) {
+      (void)hipGetLastError();
+      mgg::raise(GRUS_E_INVALID, "labels_dev is not device memory");
+    }
+    if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)
+      mgg::raise(GRUS_E_INVALID, "labels_dev is not device memory");
+    if (a.type == hipMemoryTypeDevice && a.device != g->g->ctx->device)
+      mgg::raise(GRUS_E_INVALID, "labels_dev is on another device than the graph");
+    mgg::run_cc_noalloc(*g->g, method, labels_dev, stats);
   });
 }
 

@@ -278,6 +278,22 @@ int grus_pagerank(grus_graph *g, float alpha, int iters, float *rank_out,
  * 1 = min-label propagation through the push operator (the reference's
  * algorithm). */
 int grus_cc(grus_graph *g, int method, uint32_t *labels_out, grus_stats *stats);
+/* ---- allocation-free variant (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  Computes what
+ * grus_cc computes, into the caller's device buffer labels_dev (V u32 on the
+ * graph's device), and calls no hipMalloc, hipFree or allocating routine:
+ * every other device buffer comes from a scratch area of
+ * GRUS_NOALLOC_SCRATCH_BYTES that the library holds on each device until the
+ * process ends (set up by grus_init).  Method 0 needs 4 KiB of it at any V;
+ * method 1 needs 28 (V + 2) bytes plus scan temporaries (V up to about
+ * 1.1 M); a larger graph fails with GRUS_E_BUDGET.  The call runs on the
+ * context's stream (grus_ctx_stream) and returns when it is done; no other
+ * stream may write labels_dev meanwhile (a fill on the null stream does not
+ * order with it).  Calls that overlap on one device race on the scratch
+ * area; the result is not held for grus_result_copy / grus_result_digest. */
+#define GRUS_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int grus_cc_noalloc(grus_graph *g, int method, uint32_t *labels_dev, grus_stats *stats);
 /* Copy the last app result held on the device for g (V elements of 4 B). */
 int grus_result_copy(grus_graph *g, void *out);
 /* Digest of the last u32 result held for g (BFS/SSSP/CC): reached = entries
