@@ -308,14 +308,16 @@ class Server:
     """Continuous-batching query server (grus_server_*)."""
 
     def __init__(self, g, max_inflight_batches=0, max_batch=0, state_budget_bytes=0,
-                 copy_results=False, admit_min_free=0, admit_window=0):
+                 copy_results=False, admit_min_free=0, admit_window=0, solo_max=0):
         """Zero for max_inflight_batches, max_batch, admit_min_free or
         admit_window picks the library default (4 engines per kind, 64 lanes,
         a batch admits arrivals during its first 2 steps, later only when
-        idle); admit_window < 0 turns the window off."""
+        idle; up to 8 waiting queries of a kind are served one at a time with
+        the single-source kernels); admit_window < 0 turns the window off,
+        solo_max < 0 the single-query path."""
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
-                               int(copy_results), admit_min_free, admit_window)
+                               int(copy_results), admit_min_free, admit_window, solo_max)
         h = C.c_void_p()
         capi.check(_lib().grus_server_create(g.h, C.byref(opts), C.byref(h)))
         self.h = h

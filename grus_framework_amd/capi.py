@@ -63,7 +63,7 @@ class CsrViewC(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("max_inflight_batches", C.c_int), ("max_batch", C.c_int),
                 ("state_budget_bytes", C.c_uint64), ("copy_results", C.c_int),
-                ("admit_min_free", C.c_int), ("admit_window", C.c_int)]
+                ("admit_min_free", C.c_int), ("admit_window", C.c_int), ("solo_max", C.c_int)]
 
 
 class QueryResult(C.Structure):

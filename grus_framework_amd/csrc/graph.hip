@@ -495,6 +495,42 @@ Graph::~Graph() {
   if (ctx) (void)hipSetDevice(ctx->device);
 }
 
+std::unique_ptr<Graph> graph_view(Graph &g, Context *ctx) {
+  g.csc();          // directed graphs: the in-edges, built once on g
+  g.unreachable();  // the in-degree-0 bitmap
+  auto v = std::make_unique<Graph>(ctx);
+  v->V = g.V;
+  v->E = g.E;
+  v->weighted = g.weighted;
+  v->symmetric = g.symmetric;
+  v->placement = g.placement;
+  v->split = g.split;
+  v->max_deg_vertex = g.max_deg_vertex;
+  v->max_deg = g.max_deg;
+  v->isolated = g.isolated;
+  v->host_bytes = g.host_bytes;
+  // borrowed buffers: kind None, so the view never frees them
+  auto borrow = [](Buf &dst, const Buf &src) {
+    dst.p = src.p;
+    dst.bytes = src.bytes;
+    dst.kind = Mem::None;
+  };
+  borrow(v->off, g.off);
+  borrow(v->col, g.col);
+  borrow(v->w, g.w);
+  borrow(v->col_far, g.col_far);
+  borrow(v->w_far, g.w_far);
+  borrow(v->first_col, g.first_col);
+  borrow(v->in_off, g.in_off);
+  borrow(v->in_col, g.in_col);
+  borrow(v->in_w, g.in_w);
+  v->csc_ready = g.csc_ready;
+  borrow(v->unreachable_bm, g.unreachable_bm);
+  v->unreachable_ready = true;
+  v->unreachable_count = g.unreachable_count;
+  return v;
+}
+
 // ------------------------------------------------------------------ builders
 std::unique_ptr<Graph> graph_from_host(Context *ctx, uint64_t V, uint64_t E, const uint64_t *off,
                                        const uint32_t *col, const uint32_t *w, bool symmetric,

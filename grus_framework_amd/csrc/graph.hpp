@@ -91,6 +91,9 @@ struct Context {
   int cu_count = 256;  // CUs the context's stream may use (all, or its CU mask)
   int cu_total = 256;  // CUs of the device
   bool cu_masked = false;
+  // other contexts run kernels on the device at the same time (server
+  // workers): no cooperative persistent launches, whose grid must be resident
+  bool shared_device = false;
   int wall_khz = 100000;  // device wall clock rate (wall_clock64 ticks per ms)
   grus_tuning tuning{};
   uint64_t *h_ctl = nullptr;  // pinned host mirror of the control block
@@ -191,6 +194,11 @@ void csr_from_sorted_keys(Context &ctx, const uint64_t *keys, uint64_t m, uint64
                           uint64_t *off, uint32_t *col);
 // (re)place a device-built CSR per the placement request
 void graph_apply_placement(Graph &g, const grus_place *place);
+// A graph that borrows g's arrays (CSR, CSC, first neighbours, in-degree-0
+// bitmap, all built on g first) and runs on ctx with workspaces of its own:
+// single-source apps of several threads on one graph, one view each.  g must
+// outlive the view.
+std::unique_ptr<Graph> graph_view(Graph &g, Context *ctx);
 // Upper bound of the device bytes one app (BFS, SSSP, PageRank pull, CC) on
 // a capped graph of V vertices and E edges allocates beside the graph: its
 // workspace, the result array, control blocks and scan temporaries.

@@ -673,6 +673,7 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
   if (own_stream) {
     own_ctx_ = std::make_unique<Context>(g.ctx->device);
     own_ctx_->tuning = g.ctx->tuning;
+    own_ctx_->shared_device = true;  // engines of a server run side by side
     ctx_ = own_ctx_.get();
   } else {
     ctx_ = g.ctx;

@@ -11,7 +11,10 @@
 // buffer when requested), the query is reported, and the slot is freed.
 //
 // A BFS engine holds one query per lane, an SSSP engine two (16-bit
-// distances: 128 queries per batch for the bytes of 64 in 32 bits).
+// distances: 128 queries per batch for the bytes of 64 in 32 bits).  Under
+// light load (at most solo_max queries of a kind waiting) an idle engine's
+// worker runs them one at a time with the single-source kernels on its own
+// view of the graph: a batch costs a step per level whatever it holds.
 //
 // Admission.  An idle engine takes up to a full batch.  A running engine
 // takes arrivals into its free lanes during the first admit_window steps
@@ -31,6 +34,7 @@
 #include <thread>
 #include <vector>
 
+#include "apps.hpp"
 #include "multi.hpp"
 #include "serve.hpp"
 
@@ -70,6 +74,8 @@ class ServerImpl : public Server {
       opts_.admit_window = o->admit_window;
     }
     window_ = opts_.admit_window == 0 ? 2 : std::max(0, opts_.admit_window);
+    opts_.solo_max = o ? o->solo_max : 0;
+    solo_max_ = opts_.solo_max == 0 ? 8 : std::max(0, opts_.solo_max);
     if (opts_.max_batch > 64) raise(GRUS_E_INVALID, "max_batch is at most 64");
     if (opts_.admit_min_free < 0 || opts_.admit_min_free > opts_.max_batch)
       raise(GRUS_E_INVALID, "admit_min_free is between 0 and max_batch");
@@ -95,6 +101,14 @@ class ServerImpl : public Server {
     for (int k = 0; k < 2; k++)
       for (int e = 0; e < opts_.max_inflight_batches; e++)
         engines_.push_back(std::make_unique<MsEngine>(g, (MsKind)k, lanes, true));
+    // each worker's view of the graph for single-source queries; an SSSP
+    // view builds its pull-round layout now, not in its first query
+    if (solo_max_ > 0) {
+      for (auto &e : engines_) {
+        views_.push_back(graph_view(g, &e->ctx()));
+        if (e->kind() == MsKind::SSSP) (void)sssp_binned_ready(*views_.back());
+      }
+    }
     for (size_t i = 0; i < engines_.size(); i++)
       workers_.emplace_back([this, i] { this->work(i); });
   }
@@ -157,11 +171,20 @@ class ServerImpl : public Server {
       GRUS_HIP(hipSetDevice(g_.ctx->device));
       for (;;) {
         std::vector<Query> take;
+        bool solo = false;
         {
           std::unique_lock<std::mutex> lk(mu_);
           cv_work_.wait(lk, [&] { return stop_ || !pending_[kind].empty() || eng.active(); });
           if (stop_ && !eng.active()) break;
           int room = stop_ ? 0 : eng.free_slots();
+          // light load: an idle engine serves a short queue one query at a
+          // time with the single-source kernels (a batch pays a step per
+          // level whatever it holds; one query alone is several times faster)
+          if (!stop_ && !eng.active() && !views_.empty() &&
+              (int)pending_[kind].size() <= solo_max_ && !pending_[kind].empty()) {
+            solo = true;
+            room = 1;
+          }
           // a running batch takes newcomers while it is young, or once
           // enough of its lanes are free: lanes that start together keep
           // their frontiers in step, so a step of the engine pays for one
@@ -173,6 +196,10 @@ class ServerImpl : public Server {
             take.push_back(pending_[kind].front());
             pending_[kind].pop_front();
           }
+        }
+        if (solo) {
+          serve_alone(ei, take[0]);
+          continue;
         }
         if (!take.empty()) {
           if (!eng.active()) cohort_start = eng.steps();
@@ -222,6 +249,22 @@ class ServerImpl : public Server {
     for (auto &q : orphans) finish(q, orphan_status, 0, 0);
   }
 
+  // one query with the single-source kernels on the worker's graph view
+  void serve_alone(size_t ei, Query q) {
+    Graph &v = *views_[ei];
+    Context &ctx = *v.ctx;
+    q.start_us = now_us();
+    if (q.kind == GRUS_Q_BFS) run_bfs(v, q.src, GRUS_DIR_AUTO, nullptr);
+    else run_sssp(v, q.src, nullptr);
+    uint64_t reached = 0, checksum = 0;
+    row_digest(ctx, v.result.as<uint32_t>(), v.V, &reached, &checksum);
+    if (opts_.copy_results && q.out) {
+      GRUS_HIP(hipMemcpyAsync(q.out, v.result.p, v.V * 4, hipMemcpyDeviceToHost, ctx.stream));
+      ctx.sync();
+    }
+    finish(q, GRUS_OK, reached, checksum);
+  }
+
   void fail(int code, const std::string &msg) {
     std::lock_guard<std::mutex> lk(mu_);
     failed_ = true;
@@ -233,6 +276,7 @@ class ServerImpl : public Server {
   Graph &g_;
   grus_server_opts opts_{};
   std::vector<std::unique_ptr<MsEngine>> engines_;
+  std::vector<std::unique_ptr<Graph>> views_;  // per engine: single-source queries
   std::vector<std::thread> workers_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
@@ -241,6 +285,7 @@ class ServerImpl : public Server {
   int min_free_ = 1;
   int window_ = 2;  // steps after an idle start during which a batch admits
   int lanes_ = 64;  // lanes per engine
+  int solo_max_ = 8;  // queue length up to which idle engines serve queries alone
   bool stop_ = false, failed_ = false;
   int failed_code_ = GRUS_OK;
   std::string failed_msg_;

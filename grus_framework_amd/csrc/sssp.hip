@@ -583,8 +583,9 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   // low-degree graphs (grids, road-like): one persistent cooperative launch
   static const bool no_persist = getenv("GRUS_NO_PERSISTENT") != nullptr;
   // (not under a CU mask: a cooperative grid must be co-resident, and the
-  // runtime's residency check does not know the mask)
-  const bool persistent = g.max_deg <= 32 && !no_persist && !ctx.cu_masked;
+  // runtime's residency check does not know the mask; nor beside other
+  // contexts' kernels)
+  const bool persistent = g.max_deg <= 32 && !no_persist && !ctx.cu_masked && !ctx.shared_device;
   // a round pulls when its frontier's out-edges exceed E / pull_div
   // (automatic: 16, measured best of 4..128 on RMAT s24; 0xFFFFFFFF: never)
   const uint32_t pull_div = ctx.tuning.sssp_pull_div ? ctx.tuning.sssp_pull_div : 16;

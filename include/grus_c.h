@@ -346,6 +346,10 @@ typedef struct grus_server_opts {
                                  engine started from idle, whatever
                                  admit_min_free says (0 = default 2; negative =
                                  no window) */
+  int solo_max;               /* while at most solo_max queries of a kind
+                                 wait, an idle engine's worker serves them one
+                                 at a time with the single-source kernels
+                                 (0 = default 8; negative = never) */
 } grus_server_opts;
 typedef struct grus_query_result {
   uint64_t id;
