@@ -530,11 +530,23 @@ def run_capped(grus, ctx):
             grus.bfs(g, src, copy=False)
             _, st = grus.bfs(g, src, copy=False)
             digest = g.digest()
+            # top-down only: rows read where they live (zero-copy over the
+            # link when capped), and the two-stage streaming BFS (rows
+            # gathered into a device buffer per level, then expanded)
+            _, sp_push = grus.bfs(g, src, "push", copy=False)
+            push_digest = g.digest()
+            grus.bfs_2s(g, src, copy=False)
+            _, s2 = grus.bfs_2s(g, src, copy=False)
+            s2_digest = g.digest()
             _, sp = grus.pagerank(g, 0.85, 2, copy=False)
             _, peak = ctx.device_memory()
             B = 2 * (8 * (g.V + 1) + 8 * g.E + 12 * g.V)
             res[mode] = {"bfs_ms": round(st.device_ms, 2),
                          "bfs_GTEPS": round(st.edges_reached / st.device_ms / 1e6, 2),
+                         "bfs_push_only_ms": round(sp_push.device_ms, 2),
+                         "bfs_2s_ms": round(s2.device_ms, 2),
+                         "bfs_2s_stage_edges": int(s2.reserved[0]),
+                         "push_and_2s_levels_equal": push_digest == digest and s2_digest == digest,
                          "pagerank_2it_ms": round(sp.device_ms, 1),
                          "pagerank_GBps": round(B / sp.device_ms / 1e6, 1),
                          "hbm_graph_GB": round(g.info.device_bytes / 1e9, 2),

@@ -230,6 +230,15 @@ def bfs(g, src, direction="auto", copy=True):
     return out, st
 
 
+def bfs_2s(g, src, stage_edges=0, copy=True):
+    """Two-stage top-down BFS (samples/bfs_2s/bfs_2s.cu): frontier rows are
+    gathered into a device buffer of stage_edges edges, then expanded."""
+    st = Stats()
+    out = np.zeros(g.V, np.uint32) if copy else None
+    capi.check(_lib().grus_bfs_2s(g.h, src, int(stage_edges), _p(out), C.byref(st)))
+    return out, st
+
+
 def sssp(g, src, copy=True):
     """SSSP distances (samples/sssp/sssp.cu); returns (dist or None, Stats)."""
     st = Stats()

@@ -103,6 +103,7 @@ SIGNATURES = [
     ("grus_ctx_stream", _i, [_vp, _pp]),
     ("grus_graph_destroy", _i, [_vp]),
     ("grus_bfs", _i, [_vp, _u32, _i, _vp, C.POINTER(Stats)]),
+    ("grus_bfs_2s", _i, [_vp, _u32, _u64, _vp, C.POINTER(Stats)]),
     ("grus_sssp", _i, [_vp, _u32, _vp, C.POINTER(Stats)]),
     ("grus_pagerank", _i, [_vp, _f, _i, _vp, C.POINTER(Stats)]),
     ("grus_cc", _i, [_vp, _i, _vp, C.POINTER(Stats)]),

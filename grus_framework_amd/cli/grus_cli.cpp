@@ -90,7 +90,7 @@ void die(int rc, const char *what) {
 
 void usage(const char *prog) {
   fprintf(stderr,
-          "usage: %s [--app bfs|sssp|pr|cc] (--input g.gr | --rmat S | --grid RxC)\n"
+          "usage: %s [--app bfs|bfs_2s|sssp|pr|cc] (--input g.gr | --rmat S | --grid RxC)\n"
           "  reference flags: --output FILE --src N --device N --ngpu 1 --pull\n"
           "                   --opt --pf --ab --rm --pl --wl_th X (accepted, unused)\n"
           "  placement:       --placement hbm|managed|capped --hbm_cap_gb G\n"
@@ -98,6 +98,7 @@ void usage(const char *prog) {
           "  co-location:     --cu_limit N (run on the first N compute units)\n"
           "  inputs:          --symmetric --edge_factor N --seed N --weights LO,HI --wseed N\n"
           "  apps:            --direction auto|push|pull (bfs)  --iters N --alpha A (pr)\n"
+          "                   --stage_edges N (bfs_2s: staging buffer, edges)\n"
           "                   --method hook|lp (cc)  --pr_unnormalized (dump ranks x V)\n"
           "  output:          --precise_output (ranks with 9 digits instead of 6)\n",
           prog);
@@ -111,7 +112,8 @@ int main(int argc, char **argv) {
   std::string base = prog.substr(prog.find_last_of('/') + 1);
   std::string app = f.str("app", base.rfind("grus_", 0) == 0 ? base.substr(5) : "bfs");
   if (app == "pagerank") app = "pr";
-  if (f.has("help") || (app != "bfs" && app != "sssp" && app != "pr" && app != "cc")) {
+  if (f.has("help") ||
+      (app != "bfs" && app != "bfs_2s" && app != "sssp" && app != "pr" && app != "cc")) {
     usage(argv[0]);
     return f.has("help") ? 0 : 2;
   }
@@ -195,6 +197,9 @@ int main(int argc, char **argv) {
     int dir = d == "push" ? GRUS_DIR_PUSH : d == "pull" ? GRUS_DIR_PULL : GRUS_DIR_AUTO;
     u32.resize(info.V);
     CHECK(grus_bfs(g, src, dir, u32.data(), &st));
+  } else if (app == "bfs_2s") {  // samples/bfs_2s: two-stage active-subgraph streaming
+    u32.resize(info.V);
+    CHECK(grus_bfs_2s(g, src, (uint64_t)f.num("stage_edges", 0), u32.data(), &st));
   } else if (app == "sssp") {
     u32.resize(info.V);
     CHECK(grus_sssp(g, src, u32.data(), &st));

@@ -159,6 +159,60 @@ unsigned grid_cap(Context &ctx, uint64_t items, unsigned per_block, unsigned blo
 
 unsigned push_grid(Context &ctx, uint64_t total) { return grid_cap(ctx, total, LB_CHUNK, 8); }
 
+// A workgroup copies LB_CHUNK consecutive positions: thread 0 finds the
+// queue entries that own the chunk's ends, the owners' scan values and row
+// starts are staged in LDS, and each thread copies LB_CHUNK / BLOCK edges in
+// coalesced rounds (rows are contiguous, so consecutive positions of one
+// row are consecutive reads).
+__global__ __launch_bounds__(BLOCK) void k_gather_rows(CsrView g, const uint32_t *q,
+                                                       const uint64_t *S, uint32_t F, uint64_t k0,
+                                                       uint64_t k1, uint32_t *col, uint32_t *w) {
+  __shared__ uint64_t sS[LB_WIN];
+  __shared__ uint64_t sRow[LB_WIN];
+  __shared__ uint32_t s_i0, s_n;
+  for (uint64_t c0 = k0 + (uint64_t)blockIdx.x * LB_CHUNK; c0 < k1;
+       c0 += (uint64_t)gridDim.x * LB_CHUNK) {
+    const uint64_t c1 = c0 + LB_CHUNK < k1 ? c0 + LB_CHUNK : k1;
+    if (threadIdx.x == 0) {
+      const uint32_t i0 = upper_bound_u64(S, 0, F + 1, c0) - 1;
+      const uint32_t i1 = upper_bound_u64(S, i0, F + 1, c1 - 1) - 1;
+      s_i0 = i0;
+      s_n = i1 - i0 + 2;
+    }
+    __syncthreads();
+    const uint32_t i0 = s_i0, n = s_n;
+    const bool staged = n <= LB_WIN;
+    if (staged) {
+      for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        sS[t] = S[i0 + t];
+        if (i0 + t < F) sRow[t] = g.off[q[i0 + t]];
+      }
+    }
+    __syncthreads();
+    for (uint64_t k = c0 + threadIdx.x; k < c1; k += BLOCK) {
+      uint32_t j;
+      uint64_t e;
+      if (staged) {
+        j = upper_bound_u64(sS, 0, n, k) - 1;
+        e = sRow[j] + (k - sS[j]);
+      } else {
+        j = upper_bound_u64(S + i0, 0, n, k) - 1;
+        e = g.off[q[i0 + j]] + (k - S[i0 + j]);
+      }
+      col[k - k0] = edge_col(g, e);
+      if (w) w[k - k0] = edge_w(g, e);
+    }
+    __syncthreads();  // sS / sRow reused by the next chunk
+  }
+}
+
+void gather_rows(Context &ctx, const CsrView &g, const uint32_t *q, const uint64_t *S, uint32_t F,
+                 uint64_t k0, uint64_t k1, uint32_t *col, uint32_t *w, hipStream_t stream) {
+  if (F == 0 || k1 <= k0) return;
+  k_gather_rows<<<push_grid(ctx, k1 - k0), BLOCK, 0, stream>>>(g, q, S, F, k0, k1, col, w);
+  GRUS_LAUNCH_CHECK();
+}
+
 void push_scan_degrees(Context &ctx, PushWork &w, const uint64_t *off, const uint32_t *q,
                        uint32_t F) {
   k_queue_degrees<<<grid_cap(ctx, (uint64_t)F + 1, BLOCK), BLOCK, 0, ctx.stream>>>(

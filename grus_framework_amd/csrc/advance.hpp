@@ -81,19 +81,22 @@ __device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t 
 }
 
 // Edge-balanced push over queue q[0..F) with S = exclusive scan of its
-// out-degrees (S[F] = total).  Job supplies update/generate/next-queue.
-template <class Job>
+// out-degrees (S[F] = total), for the positions [k0, k1) of the frontier's
+// edge space.  Job supplies update/generate/next-queue.  kStaged: the edges
+// of positions [k0, k1) were gathered into g.col / g.w at k - k0 (a staged
+// piece of a two-stage step); the edge id the updater sees is then k - k0.
+template <class Job, bool kStaged = false>
 __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
-                                                   const uint64_t *S, uint32_t F, uint64_t total,
-                                                   Job job, uint32_t *qn, uint32_t *qfar,
-                                                   StepCounters *cnt) {
+                                                   const uint64_t *S, uint32_t F, uint64_t k0,
+                                                   uint64_t total, Job job, uint32_t *qn,
+                                                   uint32_t *qfar, StepCounters *cnt) {
   __shared__ uint64_t sS[LB_WIN];
   __shared__ uint64_t sRow[LB_WIN];
   __shared__ uint32_t sQ[LB_WIN];
   __shared__ uint32_t s_i0, s_n;
   __shared__ uint32_t s_app[BLOCK / WAVE + 1];
   uint64_t my_edges = 0;
-  for (uint64_t c0 = (uint64_t)blockIdx.x * LB_CHUNK; c0 < total;
+  for (uint64_t c0 = k0 + (uint64_t)blockIdx.x * LB_CHUNK; c0 < total;
        c0 += (uint64_t)gridDim.x * LB_CHUNK) {
     uint64_t c1 = c0 + LB_CHUNK < total ? c0 + LB_CHUNK : total;
     if (threadIdx.x == 0) {
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
         if (i0 + t < F) {
           uint32_t u = q[i0 + t];
           sQ[t] = u;
-          sRow[t] = g.off[u];
+          if (!kStaged) sRow[t] = g.off[u];
         }
       }
     }
@@ -132,14 +135,14 @@ __global__ __launch_bounds__(BLOCK) void k_push_lb(CsrView g, const uint32_t *q,
           j = upper_bound_u64(sS, 0, n, k) - 1;
           base = sS[j];
           u = sQ[j];
-          row = sRow[j];
+          row = kStaged ? 0 : sRow[j];
         } else {
           j = upper_bound_u64(S + i0, 0, n, k) - 1;
           base = S[i0 + j];
           u = q[i0 + j];
-          row = g.off[u];
+          row = kStaged ? 0 : g.off[u];
         }
-        uint64_t e = row + (k - base);
+        const uint64_t e = kStaged ? k - k0 : row + (k - base);
         const uint32_t dst = edge_col(g, e);
         uint32_t val = 0;  // what the updater wrote (a distance), for the generator
         bool upd = job.update(u, dst, e, edge_w(g, e), &val);
@@ -269,9 +272,26 @@ void push_advance(Context &ctx, PushWork &w, const CsrView &g, const uint32_t *q
                   uint32_t *qfar = nullptr) {
   if (F == 0 || total == 0) return;
   k_push_lb<Job><<<push_grid(ctx, total), BLOCK, 0, ctx.stream>>>(
-      g, q, w.scan_ptr(), F, total, job, qn, qfar, cnt);
+      g, q, w.scan_ptr(), F, 0, total, job, qn, qfar, cnt);
   GRUS_LAUNCH_CHECK();
 }
+
+// Two-stage step (a staged piece): the frontier positions [k0, k1) whose
+// edges were gathered into stage (stage.col / stage.w at k - k0).
+template <class Job>
+void push_advance_staged(Context &ctx, const CsrView &stage, const uint32_t *q,
+                         const uint64_t *S, uint32_t F, uint64_t k0, uint64_t k1, const Job &job,
+                         uint32_t *qn, StepCounters *cnt, hipStream_t stream) {
+  if (F == 0 || k1 <= k0) return;
+  k_push_lb<Job, true><<<push_grid(ctx, k1 - k0), BLOCK, 0, stream>>>(stage, q, S, F, k0, k1, job,
+                                                                   qn, nullptr, cnt);
+  GRUS_LAUNCH_CHECK();
+}
+// positions [k0, k1) of queue q's edge space (scan S) gathered from g into
+// col / w (w may be null) at k - k0: the frontier's rows, wherever they live
+// (HBM or the host part of a capped graph), copied to a device buffer
+void gather_rows(Context &ctx, const CsrView &g, const uint32_t *q, const uint64_t *S, uint32_t F,
+                 uint64_t k0, uint64_t k1, uint32_t *col, uint32_t *w, hipStream_t stream);
 
 // a frontier of at most PUSH_SMALL_F vertices: one launch (k_push_small)
 template <class Job>

@@ -7,6 +7,9 @@ namespace mgg {
 
 // samples/bfs/bfs.cu:104-173
 void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st);
+// samples/bfs_2s/bfs_2s.cu:94-138 (two-stage: frontier rows gathered into a
+// device buffer of stage_edges edges, 0 = default, then expanded)
+void run_bfs_2s(Graph &g, uint32_t src, uint64_t stage_edges, grus_stats *st);
 // samples/sssp/sssp.cu:91-170
 void run_sssp(Graph &g, uint32_t src, grus_stats *st);
 // samples/pagerank/pr.cu:134-205 (redefined as fixed-iteration pull)

@@ -45,6 +45,28 @@ struct BfsWork {
   const void *first_for = nullptr;  // in-edge columns first_in was built from
   PushWork push;
   uint64_t cap_v = 0;
+  // two-stage steps: two staging buffers of frontier rows, the stream that
+  // fills one while the other is expanded, and the events that order them
+  Buf stage[2];
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr}, ev_scan = nullptr;
+  ~BfsWork() {
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    for (hipEvent_t e : {ev_ready[0], ev_ready[1], ev_free[0], ev_free[1], ev_scan})
+      if (e) (void)hipEventDestroy(e);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+  }
+  void ensure_stage(uint64_t edges) {
+    if (!copy_stream) {
+      GRUS_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+      for (int b = 0; b < 2; b++) {
+        GRUS_HIP(hipEventCreateWithFlags(&ev_ready[b], hipEventDisableTiming));
+        GRUS_HIP(hipEventCreateWithFlags(&ev_free[b], hipEventDisableTiming));
+      }
+      GRUS_HIP(hipEventCreateWithFlags(&ev_scan, hipEventDisableTiming));
+    }
+    for (auto &b : stage) b.ensure(std::max<uint64_t>(edges, 1) * 4);
+  }
   void ensure(uint64_t V) {
     if (V <= cap_v) return;
     uint64_t W = (V + 63) / 64;
@@ -576,6 +598,118 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
     st->vertices_reached = v_reached;
     st->edges_reached = e_reached;
     st->kernel_launches = launches;
+  }
+}
+
+// Two-stage top-down BFS: BFS_2S_single_gpu (samples/bfs_2s/bfs_2s.cu:94-138)
+// over Frontier_2S<BDF> and Subgraph<NORMAL> (include/frontier_2s.cuh:27-109,
+// include/subgraph.cuh:46-165, include/kernel_2s.cuh:36-92).  Each level
+// gathers the frontier's rows into a device buffer (Subgraph::build: there
+// a scan of the degrees, then a host memcpy per row into managed memory and
+// a prefetch) and expands them from there.  Here the gather is a kernel: the
+// degree scan of the push operator gives every frontier row its place in the
+// buffer, and a workgroup copies 1024 consecutive positions, reading rows
+// from HBM or, for a capped graph, from pinned host memory over the link.
+// The buffer holds stage_edges edges; a larger frontier is cut into pieces
+// (a row may span two), and two buffers alternate: a copy stream gathers
+// piece p+1 while the context stream expands piece p, ordered by events.
+// A vertex enters a BFS frontier once, so the reference's "local" list (the
+// previous subgraph's rows that stay active, frontier_2s.cuh:67-97) is
+// always empty here and every row is gathered once.
+void run_bfs_2s(Graph &g, uint32_t src, uint64_t stage_edges, grus_stats *st) {
+  Context &ctx = *g.ctx;
+  ctx.activate();
+  if (src >= g.V) raise(GRUS_E_INVALID, "source vertex out of range");
+  g.make_room(Graph::App::BFS);
+  if (!g.bfs) g.bfs = std::make_shared<BfsWork>();
+  BfsWork &w = *g.bfs;
+  w.ensure(g.V);
+  g.result.ensure(g.V * 4);
+  const uint64_t V = g.V, W = g.words();
+  const CsrView out = g.csr();
+  // staging capacity: as asked (default 2^26 edges, 256 MB per buffer), no
+  // more than the graph has, and within a device budget
+  uint64_t C = stage_edges ? stage_edges : (64ull << 20);
+  C = std::min<uint64_t>(C, std::max<uint64_t>(g.E, 1));
+  if (const uint64_t budget = devmem_budget(ctx.device)) {
+    const uint64_t held = devmem_live(ctx.device) + w.stage[0].bytes + w.stage[1].bytes;
+    const uint64_t margin = 16ull << 20;
+    const uint64_t room = budget > held + margin ? budget - held - margin : 0;
+    C = std::min<uint64_t>(C, room / 8);
+    if (C < 65536) raise(GRUS_E_BUDGET, "no room for the two-stage staging buffers");
+  }
+  if (w.stage[0].bytes != C * 4) {  // re-sized to this run's capacity
+    w.stage[0].release();
+    w.stage[1].release();
+  }
+  w.ensure_stage(C);
+  uint32_t *label = g.result.as<uint32_t>();
+  auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
+  uint64_t src_deg = 0;
+  {
+    uint64_t o[2];
+    GRUS_HIP(hipMemcpy(o, g.off.as<uint64_t>() + src, 16, hipMemcpyDeviceToHost));
+    src_deg = o[1] - o[0];
+  }
+  uint64_t launches = 0;
+  GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
+  GRUS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  bfs::k_init<<<grid_cap(ctx, std::max(V, W), BLOCK), BLOCK, 0, ctx.stream>>>(
+      label, V, w.visited.as<uint64_t>(), nullptr, W, src, w.q[0].as<uint32_t>());
+  GRUS_LAUNCH_CHECK();
+  launches++;
+  int cur = 0;
+  uint64_t nf = 1, mf = src_deg, v_reached = 1, e_reached = src_deg;
+  uint32_t level = 0;
+  while (nf) {
+    if (level > V) raise(GRUS_E_HIP, "bfs did not converge within V levels");
+    const uint32_t *q = w.q[cur].as<uint32_t>();
+    bfs::job_t job{label, w.visited.as<uint64_t>(), out.off, level + 1};
+    if (mf) {
+      // places of the frontier's rows in the edge space (S[nf] = mf)
+      push_scan_degrees(ctx, w.push, out.off, q, (uint32_t)nf);
+      GRUS_HIP(hipEventRecord(w.ev_scan, ctx.stream));
+      GRUS_HIP(hipStreamWaitEvent(w.copy_stream, w.ev_scan, 0));
+      const uint64_t *S = w.push.scan_ptr();
+      for (uint64_t k0 = 0, p = 0; k0 < mf; k0 += C, p++) {
+        const int b = (int)(p & 1);
+        const uint64_t k1 = std::min(mf, k0 + C);
+        uint32_t *buf = w.stage[b].as<uint32_t>();
+        // the copy stream fills buffer b once its previous piece is expanded
+        GRUS_HIP(hipStreamWaitEvent(w.copy_stream, w.ev_free[b], 0));
+        gather_rows(ctx, out, q, S, (uint32_t)nf, k0, k1, buf, nullptr, w.copy_stream);
+        GRUS_HIP(hipEventRecord(w.ev_ready[b], w.copy_stream));
+        GRUS_HIP(hipStreamWaitEvent(ctx.stream, w.ev_ready[b], 0));
+        const CsrView stage{nullptr, buf, nullptr, nullptr, nullptr, ~0ull, V, k1 - k0};
+        push_advance_staged(ctx, stage, q, S, (uint32_t)nf, k0, k1, job,
+                            w.q[cur ^ 1].as<uint32_t>(), cnt, ctx.stream);
+        GRUS_HIP(hipEventRecord(w.ev_free[b], ctx.stream));
+        launches += 2;
+      }
+      launches += 2;
+    }
+    ctx.publish_ctl(sizeof(StepCounters) / 8);  // read the step's counters and clear them
+    const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
+    nf = hc->next_count;
+    if (nf > V) raise(GRUS_E_HIP, "bfs step reported more new vertices than the graph has");
+    mf = hc->next_edges;
+    v_reached += nf;
+    e_reached += mf;
+    cur ^= 1;
+    level++;
+  }
+  GRUS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  GRUS_HIP(hipEventSynchronize(ctx.ev1));
+  float ms = 0;
+  GRUS_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+  if (st) {
+    *st = grus_stats{};
+    st->device_ms = ms;
+    st->iterations = level;
+    st->vertices_reached = v_reached;
+    st->edges_reached = e_reached;
+    st->kernel_launches = launches;
+    st->reserved[0] = C;  // staging capacity used, in edges
   }
 }
 

@@ -296,6 +296,17 @@ int grus_bfs(grus_graph *g, uint32_t src, int dir_mode, uint32_t *levels_out, gr
   });
 }
 
+int grus_bfs_2s(grus_graph *g, uint32_t src, uint64_t stage_edges, uint32_t *levels_out,
+                grus_stats *stats) {
+  return guard([&] {
+    need(g, "g");
+    g->last_result = 0;
+    mgg::run_bfs_2s(*g->g, src, stage_edges, stats);
+    g->last_result = 1;
+    copy_out(g, levels_out);
+  });
+}
+
 int grus_sssp(grus_graph *g, uint32_t src, uint32_t *dist_out, grus_stats *stats) {
   return guard([&] {
     need(g, "g");

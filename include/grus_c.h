@@ -265,6 +265,19 @@ int grus_graph_destroy(grus_graph *g);
  * kernel_pull (include/kernel.cuh:124-165). */
 int grus_bfs(grus_graph *g, uint32_t src, int dir_mode, uint32_t *levels_out,
              grus_stats *stats);
+/* Two-stage top-down BFS, the reference's active-subgraph streaming
+ * (BFS_2S_single_gpu, samples/bfs_2s/bfs_2s.cu:94-138, with Frontier_2S<BDF>,
+ * Subgraph<NORMAL> and kernel_2s, include/frontier_2s.cuh:27-109,
+ * include/subgraph.cuh:46-165, include/kernel_2s.cuh:36-92): each level's
+ * frontier rows are gathered by a kernel into a device buffer of
+ * stage_edges edges (0 = 2^26; within a device budget, what it leaves) and
+ * expanded from there; a larger frontier goes in pieces, the gather of the
+ * next overlapping the expansion of the current.  Meant for capped graphs,
+ * whose host-resident rows then cross the link once per level in bulk.
+ * stats->reserved[0] = staging capacity used (edges).  Same levels as
+ * grus_bfs. */
+int grus_bfs_2s(grus_graph *g, uint32_t src, uint64_t stage_edges, uint32_t *levels_out,
+                grus_stats *stats);
 /* SSSP distances (samples/sssp/sssp.cu:123-170; the relax is an atomic
  * min here, fixing the lost update at :54-55).  Requires weights (or unit
  * weights when the graph has none). */
