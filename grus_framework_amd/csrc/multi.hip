@@ -27,11 +27,15 @@ namespace ms {
 
 constexpr uint32_t K_PULL = 32;
 constexpr uint32_t CHUNK = 64;       // SSSP relax: out-edges per wave task (one per lane)
-constexpr int SPARSE_LANES = 16;
-#ifndef RELAX_U16_EDGES
-#define RELAX_U16_EDGES 6
+#ifndef SSSP_SPARSE_LANES
+#define SSSP_SPARSE_LANES 16
 #endif
-constexpr uint32_t RELAX_U16 = RELAX_U16_EDGES;  // 16-bit relax: edges whose CAS are in flight together     // at most this many active queries: edge-parallel relax (measured best of 4..64)
+// at most this many active lanes at a vertex: edge-parallel relax
+constexpr int SPARSE_LANES = SSSP_SPARSE_LANES;
+#ifndef RELAX_U16_EDGES
+#define RELAX_U16_EDGES 4
+#endif
+constexpr uint32_t RELAX_U16 = RELAX_U16_EDGES;  // 16-bit relax: edges whose CAS are in flight together
 static_assert(CHUNK == WAVE, "the edge-parallel relax covers a chunk with one edge per lane");
 
 struct LaneStart {
@@ -696,8 +700,13 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
     vals_.alloc((uint64_t)lanes_ * V * 4 + 8);  // level[lane][v]
     fill_u64(ctx, visit_.as<uint64_t>(), V + 1, 0);
   } else {
+#ifdef SSSP_START_WIDE
+    narrow_ = false;
+    vals_.alloc(V * 128 * 4 + 8);
+#else
     narrow_ = true;
     vals_.alloc(V * 128 * 2 + 8);  // dist[v][slot], 16 bits until a slot needs more
+#endif
   }
   fill_u64(ctx, front_.as<uint64_t>(), V + 1, 0);
   fill_u64(ctx, next_.as<uint64_t>(), V + 1, 0);
