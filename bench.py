@@ -466,7 +466,9 @@ def run_serving(grus):
     res["speedup_vs_serial"] = round(qps / serial_qps, 2)
     res["policy"] = ("4 engines per kind x 64 lanes (BFS 64 queries, SSSP 128); an idle engine "
                      "admits a batch, a running batch admits arrivals into free slots during "
-                     "its first 2 steps; each query reported when its slot finishes")
+                     "its first 2 steps; each query reported when its slot finishes; an idle "
+                     "engine serves its kind's queue one query at a time with the single-source "
+                     "kernels while that empties the queue within one batch time")
     res["continuous_8_engines_qps"] = round(qps_cont, 1)
     # one full SSSP batch alone (128 queries in 64 lanes) and one of 64:
     # the reference points for latency

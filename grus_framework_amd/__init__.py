@@ -321,9 +321,11 @@ class Server:
         """Zero for max_inflight_batches, max_batch, admit_min_free or
         admit_window picks the library default (4 engines per kind, 64 lanes,
         a batch admits arrivals during its first 2 steps, later only when
-        idle; up to 8 waiting queries of a kind are served one at a time with
-        the single-source kernels); admit_window < 0 turns the window off,
-        solo_max < 0 the single-query path."""
+        idle; waiting queries of a kind are served one at a time with the
+        single-source kernels while that would empty the queue within one
+        batch's time); solo_max > 0 fixes that queue length instead,
+        admit_window < 0 turns the window off, solo_max < 0 the single-query
+        path."""
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
                                int(copy_results), admit_min_free, admit_window, solo_max)

@@ -159,11 +159,18 @@ unsigned grid_cap(Context &ctx, uint64_t items, unsigned per_block, unsigned blo
 
 unsigned push_grid(Context &ctx, uint64_t total) { return grid_cap(ctx, total, LB_CHUNK, 8); }
 
+#ifndef GATHER_PER_CU
+#define GATHER_PER_CU 2
+#endif
+
 // A workgroup copies LB_CHUNK consecutive positions: thread 0 finds the
 // queue entries that own the chunk's ends, the owners' scan values and row
 // starts are staged in LDS, and each thread copies LB_CHUNK / BLOCK edges in
 // coalesced rounds (rows are contiguous, so consecutive positions of one
-// row are consecutive reads).
+// row are consecutive reads).  A thread issues its LB_PER reads before its
+// first write, so that many reads are in flight over the host link while
+// the grid stays small (GATHER_PER_CU workgroups per CU) and the expansion
+// of the previous piece, on another stream, finds room beside it.
 __global__ __launch_bounds__(BLOCK) void k_gather_rows(CsrView g, const uint32_t *q,
                                                        const uint64_t *S, uint32_t F, uint64_t k0,
                                                        uint64_t k1, uint32_t *col, uint32_t *w) {
@@ -189,18 +196,35 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows(CsrView g, const uint32_t
       }
     }
     __syncthreads();
-    for (uint64_t k = c0 + threadIdx.x; k < c1; k += BLOCK) {
-      uint32_t j;
-      uint64_t e;
-      if (staged) {
-        j = upper_bound_u64(sS, 0, n, k) - 1;
-        e = sRow[j] + (k - sS[j]);
-      } else {
-        j = upper_bound_u64(S + i0, 0, n, k) - 1;
-        e = g.off[q[i0 + j]] + (k - S[i0 + j]);
+    uint64_t e[LB_PER];
+    uint32_t c[LB_PER], wv[LB_PER];
+#pragma unroll
+    for (uint32_t r = 0; r < LB_PER; r++) {
+      const uint64_t k = c0 + threadIdx.x + (uint64_t)r * BLOCK;
+      e[r] = 0;
+      if (k < c1) {
+        if (staged) {
+          const uint32_t j = upper_bound_u64(sS, 0, n, k) - 1;
+          e[r] = sRow[j] + (k - sS[j]);
+        } else {
+          const uint32_t j = upper_bound_u64(S + i0, 0, n, k) - 1;
+          e[r] = g.off[q[i0 + j]] + (k - S[i0 + j]);
+        }
       }
-      col[k - k0] = edge_col(g, e);
-      if (w) w[k - k0] = edge_w(g, e);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < LB_PER; r++) {
+      const bool in = c0 + threadIdx.x + (uint64_t)r * BLOCK < c1;
+      c[r] = in ? edge_col(g, e[r]) : 0;
+      wv[r] = in && w ? edge_w(g, e[r]) : 0;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < LB_PER; r++) {
+      const uint64_t k = c0 + threadIdx.x + (uint64_t)r * BLOCK;
+      if (k < c1) {
+        col[k - k0] = c[r];
+        if (w) w[k - k0] = wv[r];
+      }
     }
     __syncthreads();  // sS / sRow reused by the next chunk
   }
@@ -209,7 +233,8 @@ __global__ __launch_bounds__(BLOCK) void k_gather_rows(CsrView g, const uint32_t
 void gather_rows(Context &ctx, const CsrView &g, const uint32_t *q, const uint64_t *S, uint32_t F,
                  uint64_t k0, uint64_t k1, uint32_t *col, uint32_t *w, hipStream_t stream) {
   if (F == 0 || k1 <= k0) return;
-  k_gather_rows<<<push_grid(ctx, k1 - k0), BLOCK, 0, stream>>>(g, q, S, F, k0, k1, col, w);
+  k_gather_rows<<<grid_cap(ctx, k1 - k0, LB_CHUNK, GATHER_PER_CU), BLOCK, 0, stream>>>(
+      g, q, S, F, k0, k1, col, w);
   GRUS_LAUNCH_CHECK();
 }
 

@@ -277,14 +277,19 @@ void push_advance(Context &ctx, PushWork &w, const CsrView &g, const uint32_t *q
 }
 
 // Two-stage step (a staged piece): the frontier positions [k0, k1) whose
-// edges were gathered into stage (stage.col / stage.w at k - k0).
+// edges were gathered into stage (stage.col / stage.w at k - k0).  The grid
+// stays below a full chip (STAGED_PER_CU workgroups per CU) so that the
+// gather of the next piece, on another stream, runs beside it.
+#ifndef STAGED_PER_CU
+#define STAGED_PER_CU 4
+#endif
 template <class Job>
 void push_advance_staged(Context &ctx, const CsrView &stage, const uint32_t *q,
                          const uint64_t *S, uint32_t F, uint64_t k0, uint64_t k1, const Job &job,
                          uint32_t *qn, StepCounters *cnt, hipStream_t stream) {
   if (F == 0 || k1 <= k0) return;
-  k_push_lb<Job, true><<<push_grid(ctx, k1 - k0), BLOCK, 0, stream>>>(stage, q, S, F, k0, k1, job,
-                                                                   qn, nullptr, cnt);
+  k_push_lb<Job, true><<<grid_cap(ctx, k1 - k0, LB_CHUNK, STAGED_PER_CU), BLOCK, 0, stream>>>(
+      stage, q, S, F, k0, k1, job, qn, nullptr, cnt);
   GRUS_LAUNCH_CHECK();
 }
 // positions [k0, k1) of queue q's edge space (scan S) gathered from g into

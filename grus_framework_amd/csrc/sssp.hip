@@ -288,7 +288,7 @@ __global__ void k_weight_sum(const uint32_t *w, uint64_t E, unsigned long long *
 struct PCtl {
   GridSync gs;
   uint32_t ring[4][4];  // per step: next count, far count, min far distance, unused
-  uint32_t rounds, stamp, status, pad;
+  uint32_t rounds, stamp, status, passes;  // passes: local passes of workgroup 0
 };
 
 struct PParams {
@@ -312,6 +312,11 @@ __device__ __forceinline__ void clear_slot(PCtl *c, uint32_t step) {
 // LDS staging of one round's output: vertices generated by a workgroup are
 // collected in LDS and published with one atomic per workgroup per round
 constexpr uint32_t PQ_CAP = 2048;
+// the ring of vertices a workgroup owes a relaxation within a round
+constexpr uint32_t PL_CAP = 4096;
+#ifndef RELAX_GROUP
+#define RELAX_GROUP 4
+#endif
 
 __device__ __forceinline__ void stage(uint32_t v, uint32_t *lq, uint32_t *lcount,
                                       uint32_t *gcount, uint32_t *gq, uint32_t goff) {
@@ -334,7 +339,7 @@ __device__ __forceinline__ void publish(const uint32_t *lq, uint32_t lcount, uin
 }
 
 __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
-  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP], s_loc[PQ_CAP];
+  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP], s_loc[PL_CAP];
   __shared__ uint32_t s_nn, s_nf, s_base, s_hi, s_nl;
   const CsrView g = p.g;
   unsigned int gen = 0;
@@ -346,6 +351,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
   uint32_t *far[2] = {p.far0, p.far1};
   int cur = 0, fcur = 0;
   bool ok = true;
+  uint32_t passes = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   while (ok) {
     uint32_t *C = p.ctl->ring[step & 3];
@@ -359,25 +365,51 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       const uint32_t Tk = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
       if (threadIdx.x == 0) s_nn = s_nf = 0;
       __syncthreads();
+      // An improved vertex below T is owed one more relaxation with its new
+      // distance.  Its improver keeps it in the workgroup's LDS ring and
+      // relaxes it in a later pass of this round, or, when the ring is full
+      // or the round's pass budget is spent, publishes it to the next
+      // round's queue (once per round: stamps).  A vertex improved again
+      // later is owed again by that improver, so at the end of a bucket
+      // every vertex below T has been relaxed with its final distance.
+      // the current pass reads ring entries from pass_lo on: an append at
+      // absolute position l is kept only while l < pass_lo + PL_CAP, so it
+      // never lands on a slot still to be read
+      uint32_t pass_lo = 0;
+      auto owe = [&](uint32_t v) {
+        const uint32_t l = atomicAdd(&s_nl, 1u);
+        if (l < pass_lo + PL_CAP) {
+          s_loc[l % PL_CAP] = v;
+        } else if (atomicExch(&p.near_stamp[v], near_id) != near_id) {
+          stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
+        }
+      };
+      // a row's edges are read first and their atomicMin issued together
+      // (RELAX_GROUP in flight): the updates of one row are independent
       auto relax = [&](uint32_t u) {
         const uint64_t e0 = g.off[u], e1 = g.off[u + 1];
         const uint32_t du = ld_agent(&p.dist[u]);
-        for (uint64_t e = e0; e < e1; e++) {
-          const uint32_t v = edge_col(g, e);
-          const uint64_t nd = (uint64_t)du + edge_w(g, e);
-          if (nd >= INFINIT || atomicMin(&p.dist[v], (uint32_t)nd) <= (uint32_t)nd) continue;
-          // generator: near below T, far otherwise, once per queue (stamps)
-          if (nd < Tk) {
-            if (atomicExch(&p.near_stamp[v], near_id) != near_id)
-              stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
-            // every improvement this workgroup makes is also worth a local
-            // hop, whoever publishes it (a full list only drops the hop)
-            if (p.local_hops) {
-              const uint32_t l = atomicAdd(&s_nl, 1u);
-              if (l < PQ_CAP) s_loc[l] = v;
+        for (uint64_t eb = e0; eb < e1; eb += RELAX_GROUP) {
+          uint32_t v[RELAX_GROUP], nd[RELAX_GROUP], old[RELAX_GROUP];
+#pragma unroll
+          for (int i = 0; i < RELAX_GROUP; i++) {
+            v[i] = 0;
+            nd[i] = INFINIT;
+            if (eb + i < e1) {
+              v[i] = edge_col(g, eb + i);
+              const uint64_t x = (uint64_t)du + edge_w(g, eb + i);
+              nd[i] = x < INFINIT ? (uint32_t)x : INFINIT;
             }
-          } else if (atomicExch(&p.far_stamp[v], far_id) != far_id) {
-            stage(v, s_far, &s_nf, &C[1], far[fcur], nfar);
+          }
+#pragma unroll
+          for (int i = 0; i < RELAX_GROUP; i++)
+            old[i] = nd[i] < INFINIT ? atomicMin(&p.dist[v[i]], nd[i]) : 0u;
+#pragma unroll
+          for (int i = 0; i < RELAX_GROUP; i++) {
+            if (nd[i] >= INFINIT || old[i] <= nd[i]) continue;
+            if (nd[i] < Tk) owe(v[i]);
+            else if (atomicExch(&p.far_stamp[v[i]], far_id) != far_id)
+              stage(v[i], s_far, &s_nf, &C[1], far[fcur], nfar);
           }
         }
       };
@@ -385,25 +417,36 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       __syncthreads();
       for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride)
         relax(ld_agent(&q[cur][i]));
-      __syncthreads();
-      // local hops: the workgroup relaxes again the vertices it has just
-      // improved (held in LDS), so the frontier advances several hops per
-      // grid barrier.  Publishing is unchanged (stamps): every improved
-      // vertex is still relaxed in the next round, so the local passes only
-      // add relaxations and never change the fixed point.  The pass count
-      // is bounded and block-uniform: thread 0 snapshots the local count
-      // between two barriers (others append while a pass runs), so every
-      // thread sees the same bound and every workgroup reaches the grid
-      // barrier.
-      for (uint32_t it = 0, lo = 0; it < p.local_hops; it++) {
-        if (threadIdx.x == 0) s_hi = s_nl < PQ_CAP ? s_nl : PQ_CAP;
+      // local passes: the workgroup relaxes the vertices it owes, so the
+      // frontier advances many hops per grid barrier.  The pass count is
+      // bounded and block-uniform: thread 0 snapshots the ring between two
+      // barriers (others append while a pass runs), so every thread sees
+      // the same bound and every workgroup reaches the grid barrier.
+      for (uint32_t it = 0, lo = 0;; it++) {
+        __syncthreads();  // the previous pass's appends done, s_hi read by all
+        if (threadIdx.x == 0) {
+          // appends that were kept all lie below pass_lo + PL_CAP
+          const uint32_t n = s_nl < pass_lo + PL_CAP ? s_nl : pass_lo + PL_CAP;
+          s_nl = n;
+          s_hi = n;
+        }
         __syncthreads();
         const uint32_t hi = s_hi;
         if (hi <= lo) break;
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) relax(s_loc[k]);
+        if (it >= p.local_hops) {  // budget spent: what is still owed goes to the next round
+          for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) {
+            const uint32_t v = s_loc[k % PL_CAP];
+            if (atomicExch(&p.near_stamp[v], near_id) != near_id)
+              stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
+          }
+          break;
+        }
+        pass_lo = lo;  // this pass reads [lo, hi)
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) relax(s_loc[k % PL_CAP]);
         lo = hi;
-        __syncthreads();  // the pass's appends done, and s_hi read by all
+        passes++;
       }
+      __syncthreads();
       publish(s_near, s_nn, &C[0], q[cur ^ 1], 0, &s_base);
       __syncthreads();
       publish(s_far, s_nf, &C[1], far[fcur], nfar, &s_base);
@@ -466,6 +509,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
     p.ctl->rounds = (uint32_t)rounds;
     p.ctl->stamp = stamp;
     p.ctl->status = ok ? 0u : 1u;
+    p.ctl->passes = passes;
   }
 }
 
@@ -499,7 +543,8 @@ uint32_t pick_delta(Graph &g, SsspWork &w) {
 
 namespace {
 // one cooperative launch for the whole traversal (low max degree)
-void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, uint64_t *rounds) {
+void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, uint64_t *rounds,
+                         uint64_t *passes) {
   Context &ctx = *g.ctx;
   const uint64_t V = g.V;
   if (w.stamp >= 0xE0000000u) {
@@ -525,9 +570,9 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   p.round_cap = std::min<uint64_t>(64 * V + 1024, 0x08000000ull);
   p.timeout = ctx.wall_ticks_per_ms() * 20000;  // 20 s per barrier wait
   p.stamp0 = w.stamp;
-  // 8 passes on the 4096^2 grid: rounds 6.5K -> 2.1K, 134 -> 130 ms; each
-  // pass costs a chain of device-scope atomics, so more passes gain little
-  p.local_hops = ctx.tuning.sssp_local_hops ? ctx.tuning.sssp_local_hops : 8;
+  // local passes per round: a bucket's frontier mostly advances within one
+  // round, and the round's barrier is paid once per bucket
+  p.local_hops = ctx.tuning.sssp_local_hops ? ctx.tuning.sssp_local_hops : 256;
   p.max_deg = (uint32_t)std::max<uint64_t>(g.max_deg, 1);
   int per_cu = 0;
   GRUS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -553,6 +598,7 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   ctx.sync();
   w.stamp = out.stamp;
   *rounds = out.rounds;
+  *passes = out.passes;
   if (out.status) raise(GRUS_E_TIMEOUT, "persistent sssp aborted (barrier timeout or round cap)");
 }
 }  // namespace
@@ -579,7 +625,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     GRUS_HIP(hipMemcpy(o, g.off.as<uint64_t>() + src, 16, hipMemcpyDeviceToHost));
     src_deg = o[1] - o[0];
   }
-  uint64_t launches = 0, rounds = 0, relaxed_vertices = 0, pull_rounds = 0;
+  uint64_t launches = 0, rounds = 0, relaxed_vertices = 0, pull_rounds = 0, local_passes = 0;
   // low-degree graphs (grids, road-like): one persistent cooperative launch
   static const bool no_persist = getenv("GRUS_NO_PERSISTENT") != nullptr;
   // (not under a CU mask: a cooperative grid must be co-resident, and the
@@ -612,7 +658,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   GRUS_LAUNCH_CHECK();
   launches++;
   if (persistent) {
-    run_sssp_persistent(g, w, src, delta, &rounds);
+    run_sssp_persistent(g, w, src, delta, &rounds, &local_passes);
     launches++;
   }
   int cur = 0, fcur = 0;
@@ -755,6 +801,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
     st->kernel_launches = launches;
     st->reserved[0] = relaxed_vertices;
     st->reserved[1] = delta;
+    st->reserved[2] = local_passes;
   }
 }
 

@@ -126,9 +126,11 @@ typedef struct grus_tuning {
                              E / sssp_pull_div; 0 = automatic (16),
                              0xFFFFFFFF = push only */
   uint32_t sssp_local_hops; /* SSSP on low-degree graphs (one persistent
-                             launch): extra passes in which a workgroup
-                             relaxes the vertices it has just improved before
-                             the grid barrier; 0 = automatic (8) */
+                             launch): at most this many passes per round in
+                             which a workgroup relaxes the vertices it has
+                             improved, before the grid barrier (what is
+                             still owed after the last pass goes to the next
+                             round); 0 = automatic (256), at most 256 */
   uint32_t pr_task_edges; /* binned PageRank layout: edges per accumulation
                              task; a bin with more edges is split over several
                              tasks (0 = automatic, 2^18) */
@@ -361,8 +363,14 @@ typedef struct grus_server_opts {
                                  no window) */
   int solo_max;               /* while at most solo_max queries of a kind
                                  wait, an idle engine's worker serves them one
-                                 at a time with the single-source kernels
-                                 (0 = default 8; negative = never) */
+                                 at a time with the single-source kernels.
+                                 0 = default: while the engines of the kind,
+                                 one query each at a time, would empty the
+                                 queue within one batch's time (running
+                                 means of both; a batch counted as 32
+                                 single queries until one has run; at most
+                                 8 waiting until a single query has run);
+                                 negative = never */
 } grus_server_opts;
 typedef struct grus_query_result {
   uint64_t id;

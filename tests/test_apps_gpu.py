@@ -496,12 +496,52 @@ def test_sssp_grid_local_hops_invariant(ctx, oracle):
         ref = oracle.sssp(host_csr(g), 5)
         t0 = ctx.tuning
         try:
-            for hops in (1, 8, 64):
+            for hops in (1, 8, 64, 256):
                 t = ctx.tuning
                 t.sssp_local_hops = hops
                 ctx.tuning = t
                 d, st = grus.sssp(g, 5)
                 print("hops %d rounds %d" % (hops, st.iterations), flush=True)
+                assert (d == ref).all()
+        finally:
+            ctx.tuning = t0
+
+
+def _binary_tree(n, wmax, seed):
+    """Symmetric CSR of the complete binary tree on n vertices (i's children
+    2i+1, 2i+2), one random weight in [1, wmax] per tree edge."""
+    rng = np.random.default_rng(seed)
+    child = np.arange(1, n, dtype=np.int64)
+    parent = (child - 1) // 2
+    wt = rng.integers(1, wmax + 1, n - 1).astype(np.uint32)
+    src = np.concatenate([parent, child])
+    dst = np.concatenate([child, parent])
+    ww = np.concatenate([wt, wt])
+    order = np.lexsort((dst, src))
+    src, dst, ww = src[order], dst[order], ww[order]
+    off = np.zeros(n + 1, dtype=np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    return off, dst.astype(np.uint32), ww
+
+
+@pytest.mark.parametrize("wmax", [1, 40])
+def test_sssp_persistent_ring_spill(ctx, oracle, wmax):
+    # a binary tree has degree <= 3 (the persistent path) and a frontier that
+    # doubles per hop: the workgroup holding the source owes far more
+    # vertices than its LDS ring holds, so owed vertices spill to the next
+    # round's queue; distances still equal Dijkstra's, for every pass budget
+    off, col, w = _binary_tree((1 << 17) - 1, wmax, 7)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=True) as g:
+        ref = oracle.sssp(host_csr(g), 0)
+        t0 = ctx.tuning
+        try:
+            for hops in (0, 3):
+                t = ctx.tuning
+                t.sssp_local_hops = hops
+                ctx.tuning = t
+                d, st = grus.sssp(g, 0)
+                print("wmax %d hops %d rounds %d" % (wmax, hops, st.iterations), flush=True)
                 assert (d == ref).all()
         finally:
             ctx.tuning = t0
