@@ -25,6 +25,8 @@ namespace mgg {
 
 struct SsspWork {
   Buf q[2], far[2], near_stamp, far_stamp, pctl;
+  Buf pad;               // persistent path: padded rows (see k_pad_rows)
+  uint32_t pad_deg = 0;  // slots per padded row (0 = not built)
   Buf front_bm;               // pull rounds: frontier bitmap
   Buf win_row, head_min, tail_min;  // pull rounds: window table of the in-edges
   const void *win_for = nullptr;    // in-edge offsets the table was built on
@@ -293,6 +295,8 @@ struct PCtl {
 
 struct PParams {
   CsrView g;
+  const uint2 *pad;  // padded rows of pad_deg slots {col, w}, or null (CSR)
+  uint32_t pad_deg;
   uint32_t *dist, *q0, *q1, *far0, *far1, *near_stamp, *far_stamp;
   PCtl *ctl;
   uint64_t delta, round_cap, timeout;
@@ -317,6 +321,8 @@ constexpr uint32_t PL_CAP = 4096;
 #ifndef RELAX_GROUP
 #define RELAX_GROUP 4
 #endif
+static_assert(RELAX_GROUP == 2 || RELAX_GROUP == 4,
+              "a padded row (4 or 8 slots) is read in whole groups of two-slot loads");
 
 __device__ __forceinline__ void stage(uint32_t v, uint32_t *lq, uint32_t *lcount,
                                       uint32_t *gcount, uint32_t *gq, uint32_t goff) {
@@ -338,8 +344,21 @@ __device__ __forceinline__ void publish(const uint32_t *lq, uint32_t lcount, uin
   for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) st_agent(&gq[goff + base + k], lq[k]);
 }
 
+// Padded rows for low-degree graphs: row v holds D slots {col, w} at v * D
+// (unused slots weigh INFINIT, so they never relax), so a relaxation reads
+// its edges with no offset lookup first: one dependent load fewer per hop.
+__global__ __launch_bounds__(BLOCK) void k_pad_rows(CsrView g, uint32_t D, uint2 *pad) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t v = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; v < g.V; v += stride) {
+    const uint64_t e0 = g.off[v], d = g.off[v + 1] - e0;
+    for (uint32_t i = 0; i < D; i++)
+      pad[v * D + i] = i < d ? make_uint2(edge_col(g, e0 + i), edge_w(g, e0 + i))
+                             : make_uint2(0u, INFINIT);
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
-  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP], s_loc[PL_CAP];
+  __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP], s_loc[PL_CAP], s_locd[PL_CAP];
   __shared__ uint32_t s_nn, s_nf, s_base, s_hi, s_nl;
   const CsrView g = p.g;
   unsigned int gen = 0;
@@ -376,21 +395,52 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       // absolute position l is kept only while l < pass_lo + PL_CAP, so it
       // never lands on a slot still to be read
       uint32_t pass_lo = 0;
-      auto owe = [&](uint32_t v) {
+      auto owe = [&](uint32_t v, uint32_t dv) {
         const uint32_t l = atomicAdd(&s_nl, 1u);
         if (l < pass_lo + PL_CAP) {
           s_loc[l % PL_CAP] = v;
+          s_locd[l % PL_CAP] = dv;  // the distance it is owed a relaxation with
         } else if (atomicExch(&p.near_stamp[v], near_id) != near_id) {
           stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
         }
       };
-      // a row's edges are read first and their atomicMin issued together
-      // (RELAX_GROUP in flight): the updates of one row are independent
-      auto relax = [&](uint32_t u) {
+      // the atomicMin of up to RELAX_GROUP edges issued together (the
+      // updates of one row are independent), then their outcomes
+      auto settle = [&](const uint32_t *v, const uint32_t *nd) {
+        uint32_t old[RELAX_GROUP];
+#pragma unroll
+        for (int i = 0; i < RELAX_GROUP; i++)
+          old[i] = nd[i] < INFINIT ? atomicMin(&p.dist[v[i]], nd[i]) : 0u;
+#pragma unroll
+        for (int i = 0; i < RELAX_GROUP; i++) {
+          if (nd[i] >= INFINIT || old[i] <= nd[i]) continue;
+          if (nd[i] < Tk) owe(v[i], nd[i]);
+          else if (atomicExch(&p.far_stamp[v[i]], far_id) != far_id)
+            stage(v[i], s_far, &s_nf, &C[1], far[fcur], nfar);
+        }
+      };
+      // relax u's out-edges with distance du (u's distance when it was
+      // improved; a later improvement is owed by its own improver)
+      auto relax = [&](uint32_t u, uint32_t du) {
+        uint32_t v[RELAX_GROUP], nd[RELAX_GROUP];
+        if (p.pad) {
+          const uint4 *row = reinterpret_cast<const uint4 *>(p.pad + (uint64_t)u * p.pad_deg);
+          for (uint32_t b = 0; b < p.pad_deg; b += RELAX_GROUP) {
+#pragma unroll
+            for (int i = 0; i < RELAX_GROUP; i += 2) {
+              const uint4 x = row[(b + i) / 2];
+              v[i] = x.x;
+              v[i + 1] = x.z;
+              const uint64_t a = (uint64_t)du + x.y, c = (uint64_t)du + x.w;
+              nd[i] = a < INFINIT ? (uint32_t)a : INFINIT;
+              nd[i + 1] = c < INFINIT ? (uint32_t)c : INFINIT;
+            }
+            settle(v, nd);
+          }
+          return;
+        }
         const uint64_t e0 = g.off[u], e1 = g.off[u + 1];
-        const uint32_t du = ld_agent(&p.dist[u]);
         for (uint64_t eb = e0; eb < e1; eb += RELAX_GROUP) {
-          uint32_t v[RELAX_GROUP], nd[RELAX_GROUP], old[RELAX_GROUP];
 #pragma unroll
           for (int i = 0; i < RELAX_GROUP; i++) {
             v[i] = 0;
@@ -401,22 +451,15 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
               nd[i] = x < INFINIT ? (uint32_t)x : INFINIT;
             }
           }
-#pragma unroll
-          for (int i = 0; i < RELAX_GROUP; i++)
-            old[i] = nd[i] < INFINIT ? atomicMin(&p.dist[v[i]], nd[i]) : 0u;
-#pragma unroll
-          for (int i = 0; i < RELAX_GROUP; i++) {
-            if (nd[i] >= INFINIT || old[i] <= nd[i]) continue;
-            if (nd[i] < Tk) owe(v[i]);
-            else if (atomicExch(&p.far_stamp[v[i]], far_id) != far_id)
-              stage(v[i], s_far, &s_nf, &C[1], far[fcur], nfar);
-          }
+          settle(v, nd);
         }
       };
       if (threadIdx.x == 0) s_nl = 0;
       __syncthreads();
-      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride)
-        relax(ld_agent(&q[cur][i]));
+      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride) {
+        const uint32_t u = ld_agent(&q[cur][i]);
+        relax(u, ld_agent(&p.dist[u]));
+      }
       // local passes: the workgroup relaxes the vertices it owes, so the
       // frontier advances many hops per grid barrier.  The pass count is
       // bounded and block-uniform: thread 0 snapshots the ring between two
@@ -442,7 +485,8 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
           break;
         }
         pass_lo = lo;  // this pass reads [lo, hi)
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK) relax(s_loc[k % PL_CAP]);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += BLOCK)
+          relax(s_loc[k % PL_CAP], s_locd[k % PL_CAP]);
         lo = hi;
         passes++;
       }
@@ -574,6 +618,22 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
   // round, and the round's barrier is paid once per bucket
   p.local_hops = ctx.tuning.sssp_local_hops ? ctx.tuning.sssp_local_hops : 256;
   p.max_deg = (uint32_t)std::max<uint64_t>(g.max_deg, 1);
+  // padded rows where the maximum degree is at most 8 (built once per graph;
+  // without room under a device budget the CSR rows are read instead)
+  if (g.max_deg <= 8 && !w.pad_deg) {
+    const uint32_t D = g.max_deg <= 4 ? 4 : 8;
+    try {
+      w.pad.alloc(V * D * 8);
+      sssp::k_pad_rows<<<grid_cap(ctx, V, BLOCK), BLOCK, 0, ctx.stream>>>(g.csr(), D,
+                                                                         w.pad.as<uint2>());
+      GRUS_LAUNCH_CHECK();
+      w.pad_deg = D;
+    } catch (const Error &e) {
+      if (e.code != GRUS_E_BUDGET) throw;
+    }
+  }
+  p.pad = w.pad_deg ? w.pad.as<uint2>() : nullptr;
+  p.pad_deg = w.pad_deg;
   int per_cu = 0;
   GRUS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void *>(sssp::k_persistent), BLOCK, 0));

@@ -96,7 +96,7 @@ void autotune(Graph &g, int kind, const uint32_t *srcs, int n, grus_tuning *out)
     sweep<uint32_t>(g, kind, s, &grus_tuning::sssp_pull_div, {4, 8, 16, 64, 128, 0xFFFFFFFFu},
                     &best);
     if (g.max_deg <= 32)  // the persistent path
-      sweep<uint32_t>(g, kind, s, &grus_tuning::sssp_local_hops, {1, 2, 4, 16}, &best);
+      sweep<uint32_t>(g, kind, s, &grus_tuning::sssp_local_hops, {8, 32, 256}, &best);
   } else {
     // the relabelled layout is left out: it only pays on graphs whose
     // numbering scatters the hubs, and it costs a full relabelling build
