@@ -76,9 +76,11 @@ class Regions:
     def begin(self, name):
         self.windows.append({"name": name, "begin": self._now()})
 
-    def end(self, units):
+    def end(self, units, config=None):
         self.windows[-1]["end"] = self._now()
         self.windows[-1]["units"] = units
+        if config is not None:  # settings the window ran with (kept with its traffic)
+            self.windows[-1]["config"] = config
 
     def write(self, path):
         if path:
@@ -112,11 +114,15 @@ def copy_calibration():
     log("copy calibration: 1 GiB device-to-device copy done")
 
 
-def traffic(key):
-    """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled)."""
+def traffic(key, config=None):
+    """Measured HBM bytes per unit of a timed window (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE, scaled), or None when not recorded or when it was recorded
+    with other settings than `config` (the window's settings in this run)."""
     try:
         with open(TRAFFIC) as f:
             t = json.load(f)[key]
+        if config is not None and t.get("config") != config:
+            return None
         return int(t["fetch_bytes"] + t["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
@@ -191,7 +197,9 @@ def main():
         vr += st.vertices_reached
         dev_ms += st.device_ms
     wall = time.perf_counter() - t0
-    REGIONS.end(args.steps)
+    tc = ctx.tuning
+    bfs_config = {"bfs_tuning": [tc.bfs_alpha, tc.bfs_beta, tc.bu_thread_edges]}
+    REGIONS.end(args.steps, bfs_config)
     gteps = er / wall / 1e9
     algo_bytes = (4 * er + 16 * vr + 4 * g.V * args.steps) / args.steps
     achieved = algo_bytes / (dev_ms / args.steps * 1e-3) / 1e9
@@ -251,7 +259,8 @@ def main():
                                "dominates)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic("bfs_rmat_s24_traversal"),
+                     "traffic": traffic("bfs_rmat_s%d_traversal" % args.scale, bfs_config),
+                     "traffic_config": bfs_config,
                      "algorithmic_bytes": int(algo_bytes),
                      "device_ms": round(dev_ms / args.steps, 4)},
         "cpu_baseline": cpu,

@@ -67,12 +67,14 @@ struct BfsWork {
     }
     for (auto &b : stage) b.ensure(std::max<uint64_t>(edges, 1) * 4);
   }
-  void ensure(uint64_t V) {
+  void ensure(Context &ctx, uint64_t V) {
     if (V <= cap_v) return;
     uint64_t W = (V + 63) / 64;
     visited.alloc((W + 1) * 8);
     stamp.alloc(W * 64 + 64);
-    GRUS_HIP(hipMemset(stamp.p, 0, W * 64 + 64));
+    // cleared on the context's stream, ahead of the kernels that read it
+    // (a null-stream memset is not ordered with that non-blocking stream)
+    fill_u32(ctx, stamp.as<uint32_t>(), W * 16 + 16, 0);
     last_stamp = 0;
     front[0].alloc((W + 1) * 8);
     front[1].alloc((W + 1) * 8);
@@ -414,7 +416,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
   g.make_room(Graph::App::BFS);
   if (!g.bfs) g.bfs = std::make_shared<BfsWork>();
   BfsWork &w = *g.bfs;
-  w.ensure(g.V);
+  w.ensure(ctx, g.V);
   g.result.ensure(g.V * 4);
   const uint64_t V = g.V, W = g.words();
   const CsrView out = g.csr();
@@ -623,7 +625,7 @@ void run_bfs_2s(Graph &g, uint32_t src, uint64_t stage_edges, grus_stats *st) {
   g.make_room(Graph::App::BFS);
   if (!g.bfs) g.bfs = std::make_shared<BfsWork>();
   BfsWork &w = *g.bfs;
-  w.ensure(g.V);
+  w.ensure(ctx, g.V);
   g.result.ensure(g.V * 4);
   const uint64_t V = g.V, W = g.words();
   const CsrView out = g.csr();

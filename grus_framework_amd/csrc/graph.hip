@@ -132,7 +132,8 @@ Context::Context(int dev) {
   GRUS_HIP(hipEventCreate(&ev1));
   GRUS_HIP(hipHostMalloc((void **)&h_ctl, 4096, hipHostMallocDefault));
   d_ctl.alloc(4096);
-  GRUS_HIP(hipMemset(d_ctl.p, 0, 4096));
+  GRUS_HIP(hipMemsetAsync(d_ctl.p, 0, 4096, stream));
+  GRUS_HIP(hipStreamSynchronize(stream));
   tuning.bfs_alpha = 14.0;
   tuning.bfs_beta = 24.0;
   tuning.bu_thread_edges = 32;

@@ -19,7 +19,8 @@ automatically.
             (counter KiB * 1024), so bytes = KiB * 1024 * factor.  With
             --traffic-out the scaled bytes per unit of every other window
             are merged into that JSON file as <window>.fetch_bytes /
-            .write_bytes.
+            .write_bytes, with the settings the window ran with (its
+            "config" in R.json, when bench.py recorded one).
 Output is markdown on stdout.
 """
 import argparse
@@ -128,6 +129,8 @@ def summarise_counters(regions, rows, clock, traffic_out):
             e = merged.setdefault(r["name"], {})
             e[field + "_bytes"] = int(round(kib / units * 1024 * factor))
             e[field + "_factor"] = round(factor, 4)
+            if "config" in r:
+                e["config"] = r["config"]
     if traffic_out and factor:
         with open(traffic_out, "w") as f:
             json.dump(merged, f, indent=1, sort_keys=True)

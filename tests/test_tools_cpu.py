@@ -83,3 +83,32 @@ def test_rocprof_summary_counter_calibration(tmp_path):
     assert t["pr_iter"]["fetch_bytes"] == 100 * 1024 * 2
     assert t["pr_iter"]["write_bytes"] == 7  # merged, not replaced
     assert "k_late" not in text
+
+
+def test_traffic_kept_with_window_settings(tmp_path, monkeypatch):
+    # a window's recorded settings travel into traffic.json, and bench.py
+    # reports the traffic only for a run with the same settings
+    import json
+    import sys
+    from grus_framework_amd import rocprof_summary as rs
+    _regions(tmp_path / "r.json", [("copy_1GiB", 0, 100, 1), ("bfs_w", 200, 400, 2)])
+    r = json.loads((tmp_path / "r.json").read_text())
+    r[1]["config"] = {"bfs_tuning": [14.0, 64.0, 32]}
+    (tmp_path / "r.json").write_text(json.dumps(r))
+    rows = ["Kernel_Name,Counter_Name,Counter_Value,Start_Timestamp,End_Timestamp",
+            "copy,WRITE_SIZE,1048576,10,20",
+            "k_pull(x),WRITE_SIZE,10,210,220"]
+    (tmp_path / "c.csv").write_text("\n".join(rows) + "\n")
+    out = tmp_path / "traffic.json"
+    out.write_text(json.dumps({"bfs_w": {"fetch_bytes": 100}}))
+    rs.summarise_counters(rs.load_regions(str(tmp_path / "r.json")),
+                          rs.load_rows(str(tmp_path / "c.csv")), "monotonic", str(out))
+    t = json.loads(out.read_text())
+    assert t["bfs_w"]["config"] == {"bfs_tuning": [14.0, 64.0, 32]}
+    assert t["bfs_w"]["write_bytes"] == 5 * 1024
+    sys.path.insert(0, ROOT)
+    import bench
+    monkeypatch.setattr(bench, "TRAFFIC", str(out))
+    assert bench.traffic("bfs_w", {"bfs_tuning": [14.0, 64.0, 32]}) == 100 + 5 * 1024
+    assert bench.traffic("bfs_w", {"bfs_tuning": [14.0, 24.0, 32]}) is None
+    assert bench.traffic("bfs_w") == 100 + 5 * 1024
