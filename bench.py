@@ -7,7 +7,10 @@ Headline workload (one step = one BFS traversal from one source): RMAT scale
 self-loops dropped (V = 16,777,216, E = 520,761,196 directed edges), built
 on the device and resident in HBM before timing.  Sources: the max-degree
 vertex and 16 seeded random non-isolated vertices, cycled; the direction
-thresholds are autotuned first on a different seeded sample.  value =
+thresholds are the library defaults (alpha 14, beta 64, 32 bottom-up edges
+per lane: what the autotuner picks on this graph in most runs, pinned so
+the measured HBM traffic in profiles/traffic.json belongs to the same
+schedule; --autotune tunes them first on a different seeded sample).  value =
 GTEPS = sum over the timed steps of E_r (directed edges of the reached
 component) / wall time of the timed loop (each call returns when the device
 is idle).
@@ -139,7 +142,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only (no SSSP / PageRank / CC / serving / capped)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default thresholds")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default thresholds (the default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the BFS thresholds on another sample of sources first")
     ap.add_argument("--bfs-tuning", default="",
                     help="alpha,beta,cutoff: use these BFS thresholds instead of autotuning "
                          "(e.g. the values an unprofiled run's autotuner printed)")
@@ -178,7 +184,7 @@ def main():
         ctx.tuning = t
         tuned = {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta,
                  "bu_thread_edges": t.bu_thread_edges, "source": "--bfs-tuning"}
-    elif not args.no_tune:
+    elif args.autotune and not args.no_tune:
         tune_srcs = pick_sources(deg, 8, seed=99)[1:]
         t = grus.autotune(g, "bfs", tune_srcs)
         tuned = {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta,
@@ -236,6 +242,7 @@ def main():
         except grus.GrusError as e:  # e.g. not enough device or host memory
             extra["capped_rmat_s27"] = {"error": str(e)}
 
+    bfs_traffic = traffic("bfs_rmat_s%d_traversal" % args.scale, bfs_config)
     line = {
         "metric": "BFS GTEPS at RMAT s24 (directed edges of the reached component / s); "
                   "SSSP GTEPS, PageRank GB/s, serving queries/s in extra",
@@ -253,14 +260,17 @@ def main():
         "config": {"workload": "bfs_rmat_s%d_ef16" % args.scale, "V": int(g.V), "E": int(h.E),
                    "sources": len(srcs),
                    "direction": ("auto (" + ("pinned" if args.bfs_tuning else "autotuned") + ")")
-                   if tuned else "auto",
+                   if tuned else "auto (library defaults)",
                    "tuning": tuned, "graph500_gteps": round(gteps / 2, 3)},
         "roofline": {"kernel": "BFS traversal of one source (all its launches; k_pull "
                                "dominates)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic("bfs_rmat_s%d_traversal" % args.scale, bfs_config),
+                     "traffic": bfs_traffic,
                      "traffic_config": bfs_config,
+                     # HBM bytes the counters saw per traversal / its device time
+                     "frac_counters": round(bfs_traffic / (dev_ms / args.steps * 1e-3) / 1e9
+                                            / HBM_PEAK_GBS, 4) if bfs_traffic else None,
                      "algorithmic_bytes": int(algo_bytes),
                      "device_ms": round(dev_ms / args.steps, 4)},
         "cpu_baseline": cpu,

@@ -443,7 +443,7 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
   uint32_t *label = g.result.as<uint32_t>();
   auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
   const double alpha = ctx.tuning.bfs_alpha > 0 ? ctx.tuning.bfs_alpha : 14.0;
-  const double beta = ctx.tuning.bfs_beta > 0 ? ctx.tuning.bfs_beta : 24.0;
+  const double beta = ctx.tuning.bfs_beta > 0 ? ctx.tuning.bfs_beta : 64.0;
   const uint32_t K = ctx.tuning.bu_thread_edges ? ctx.tuning.bu_thread_edges : 32;
 
   uint64_t src_deg = 0;

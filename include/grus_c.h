@@ -101,7 +101,7 @@ typedef struct grus_stats {
 /* Tunables of the traversal engine (the autotuner writes these). */
 typedef struct grus_tuning {
   double bfs_alpha;       /* push->pull when m_f > m_u / alpha   (default 14) */
-  double bfs_beta;        /* pull->push when n_f < V / beta      (default 24) */
+  double bfs_beta;        /* pull->push when n_f < V / beta      (default 64) */
   uint32_t bu_thread_edges; /* bottom-up: edges a lane scans before handing
                                the vertex to a wave (default 32) */
   uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic) */
