@@ -6,7 +6,6 @@
 // scan), so RMAT scale 24 is built in well under a second instead of the
 // reference's host-side serial loops (csr_tocsc, include/graph.cuh:273-303).
 #include <algorithm>
-#include <chrono>
 #include <atomic>
 #include <cstring>
 
@@ -132,8 +131,6 @@ Context::Context(int dev) {
   GRUS_HIP(hipEventCreate(&ev0));
   GRUS_HIP(hipEventCreate(&ev1));
   GRUS_HIP(hipHostMalloc((void **)&h_ctl, 4096, hipHostMallocDefault));
-  GRUS_HIP(hipHostMalloc((void **)&h_seq, 64, hipHostMallocDefault));
-  *h_seq = 0;
   d_ctl.alloc(4096);
   GRUS_HIP(hipMemsetAsync(d_ctl.p, 0, 4096, stream));
   GRUS_HIP(hipStreamSynchronize(stream));
@@ -149,7 +146,6 @@ Context::~Context() {
   d_ctl.release();
   scan_tmp.release();
   if (h_ctl) (void)hipHostFree(h_ctl);
-  if (h_seq) (void)hipHostFree(h_seq);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -176,35 +172,20 @@ void Context::set_cu_mask(const uint32_t *mask, uint32_t words) {
 }
 
 namespace {
-// host polling time before falling back to a stream synchronisation
-constexpr double kPublishSpinUs = 2000;
-
-__global__ void k_publish_ctl(uint64_t *d, uint64_t *h, uint32_t n, uint64_t *hs, uint64_t seq) {
+__global__ void k_publish_ctl(uint64_t *d, uint64_t *h, uint32_t n) {
   const uint32_t i = threadIdx.x;
   if (i < n) {
     h[i] = d[i];
     d[i] = 0;
   }
-  __threadfence_system();  // each lane's copy visible to the host ...
-  __syncthreads();
-  // ... before the sequence word that tells the host they are there
-  if (i == 0) __hip_atomic_store(hs, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 }  // namespace
 
 void Context::publish_ctl(size_t n) {
   if (n > 64) raise(GRUS_E_INVALID, "publish_ctl: at most 64 words");
-  const uint64_t seq = ++ctl_seq;
-  k_publish_ctl<<<1, 64, 0, stream>>>(d_ctl.as<uint64_t>(), h_ctl, (uint32_t)n, h_seq, seq);
+  k_publish_ctl<<<1, 64, 0, stream>>>(d_ctl.as<uint64_t>(), h_ctl, (uint32_t)n);
   GRUS_LAUNCH_CHECK();
-  const auto t0 = std::chrono::steady_clock::now();
-  while (__atomic_load_n(h_seq, __ATOMIC_ACQUIRE) != seq) {
-    if (std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() >
-        kPublishSpinUs) {
-      GRUS_HIP(hipStreamSynchronize(stream));  // the copy is complete once the stream is
-      break;
-    }
-  }
+  GRUS_HIP(hipStreamSynchronize(stream));
 }
 
 void Context::read_ctl(size_t n) {

@@ -97,7 +97,6 @@ struct Context {
   int wall_khz = 100000;  // device wall clock rate (wall_clock64 ticks per ms)
   grus_tuning tuning{};
   uint64_t *h_ctl = nullptr;  // pinned host mirror of the control block
-  uint64_t *h_seq = nullptr;  // pinned word publish_ctl's kernel writes last
   Buf d_ctl;                  // device control block (counters)
   Buf scan_tmp;               // rocPRIM temporary storage, grown on demand
   explicit Context(int dev);
@@ -108,13 +107,8 @@ struct Context {
   void read_ctl(size_t n);
   // one kernel copies the first n u64 of the control block to h_ctl and
   // zeroes them on the device; then wait.  Replaces a copy and the next
-  // step's memset (two launches) in per-level read-backs.  The wait polls
-  // a sequence word (h_seq) the kernel writes last, with a system-scope
-  // release, which the host sees sooner than the stream's completion;
-  // after kPublishSpinUs without it, the stream is synchronised instead
-  // (which also reports a failed launch).
+  // step's memset (two launches) in per-level read-backs.
   void publish_ctl(size_t n);
-  uint64_t ctl_seq = 0;  // last sequence word publish_ctl asked for
   // replace the stream by one restricted to the CUs set in mask (words == 0:
   // all CUs); waits for the old stream first
   void set_cu_mask(const uint32_t *mask, uint32_t words);
