@@ -25,6 +25,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -94,6 +95,63 @@ class Regions:
 REGIONS = Regions()
 
 
+class Watchdog:
+    """Guards the extra legs after the headline is measured.
+
+    Each leg announces itself (`leg`), which logs its name.  A leg that has
+    not finished after `limit_s` seconds (a stuck device call cannot be
+    interrupted from Python) is abandoned: the JSON line is printed with
+    that leg marked as having no result, and the process exits, so the
+    headline and the legs that finished are still reported."""
+
+    def __init__(self, limit_s):
+        self.limit_s = limit_s
+        self.lock = threading.Lock()
+        self.printed = False
+        self.line = None
+        self.name = None
+        self.t0 = time.time()
+        if limit_s > 0:
+            threading.Thread(target=self._watch, daemon=True).start()
+
+    def leg(self, name):
+        self.name, self.t0 = name, time.time()
+        log("leg %s" % name)
+
+    def emit(self, line):
+        with self.lock:
+            if not self.printed:
+                self.printed = True
+                print(json.dumps(line), flush=True)
+
+    def _watch(self):
+        while True:
+            time.sleep(2)
+            line, name = self.line, self.name
+            if line is None or name is None or time.time() - self.t0 < self.limit_s:
+                continue
+            # the main thread is stuck inside the leg, so these copies do not
+            # race with it
+            line = dict(line)
+            line["extra"] = dict(line["extra"])
+            line["extra"][name] = {"error": "no result within %d s; leg abandoned" %
+                                            self.limit_s}
+            log("leg %s: no result within %d s, reporting without it" % (name, self.limit_s))
+            finish_line(line)
+            self.emit(line)
+            REGIONS.write(ARGS_REGIONS_OUT[0])
+            os._exit(0)
+
+
+def finish_line(line):
+    extra = line["extra"]
+    if "pagerank" in extra and "roofline" in extra["pagerank"]:
+        line["roofline_pagerank"] = extra["pagerank"].pop("roofline")
+
+
+ARGS_REGIONS_OUT = [""]
+
+
 def copy_calibration():
     """One plain device-to-device copy of exactly 1 GiB inside a profiled
     region: FETCH_SIZE / WRITE_SIZE of this dispatch give the factors that
@@ -152,6 +210,9 @@ def main():
     ap.add_argument("--only", default="",
                     help="comma list of extras to run: pagerank,cc,sssp (or sssp_rmat, "
                          "sssp_grid),cu,serving,capped")
+    ap.add_argument("--leg-limit", type=float, default=240.0,
+                    help="seconds after which an extra leg with no result is abandoned "
+                         "(0 = never)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device-to-device copy in its own timed window")
     ap.add_argument("--regions-out", default="",
@@ -231,17 +292,6 @@ def main():
                          "the %d sources on the full graph, %s" % (n_cpu, len(srcs), cpu_model())}
 
     extra = {}
-    if not args.no_extra:
-        extra = run_extra(grus, O, ctx, g, h, threads, args)
-    g.close()
-    # the capped leg caps the whole device: it runs with no other graph held
-    only = set(x for x in args.only.split(",") if x)
-    if not args.no_extra and (not only or "capped" in only):
-        try:
-            extra["capped_rmat_s27"] = run_capped(grus, ctx)
-        except grus.GrusError as e:  # e.g. not enough device or host memory
-            extra["capped_rmat_s27"] = {"error": str(e)}
-
     bfs_traffic = traffic("bfs_rmat_s%d_traversal" % args.scale, bfs_config)
     line = {
         "metric": "BFS GTEPS at RMAT s24 (directed edges of the reached component / s); "
@@ -277,32 +327,49 @@ def main():
         "gpu_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "extra": extra,
     }
-    if "pagerank" in extra:
-        line["roofline_pagerank"] = extra["pagerank"].pop("roofline")
-    print(json.dumps(line), flush=True)
+    ARGS_REGIONS_OUT[0] = args.regions_out
+    watch = Watchdog(args.leg_limit if not args.no_extra else 0)
+    watch.line = line
+    if not args.no_extra:
+        run_extra(grus, O, ctx, g, h, threads, args, extra, watch)
+    g.close()
+    # the capped leg caps the whole device: it runs with no other graph held
+    only = set(x for x in args.only.split(",") if x)
+    if not args.no_extra and (not only or "capped" in only):
+        watch.leg("capped_rmat_s27")
+        try:
+            extra["capped_rmat_s27"] = run_capped(grus, ctx)
+        except grus.GrusError as e:  # e.g. not enough device or host memory
+            extra["capped_rmat_s27"] = {"error": str(e)}
+    watch.name = None  # every leg done
+    finish_line(line)
+    watch.emit(line)
     REGIONS.write(args.regions_out)
     ctx.close()
 
 
-def run_extra(grus, O, ctx, g, h, threads, args):
-    out = {}
+def run_extra(grus, O, ctx, g, h, threads, args, out, watch):
     only = set(x for x in args.only.split(",") if x)
 
     def want(name):
         return not only or name in only
 
     if want("pagerank"):
+        watch.leg("pagerank")
         run_pagerank(grus, O, g, h, threads, args, out)
     if want("cc"):
+        watch.leg("cc")
         run_cc(grus, O, g, h, threads, args, out)
     if want("sssp") or want("sssp_rmat") or want("sssp_grid"):
+        watch.leg("sssp")
         run_sssp(grus, O, ctx, threads, args, out,
                  rmat=want("sssp") or want("sssp_rmat"), grid=want("sssp") or want("sssp_grid"))
     if want("serving"):
+        watch.leg("serving_rmat_s22")
         out["serving_rmat_s22"] = run_serving(grus)
     if want("cu"):
+        watch.leg("bfs_cu_scaling")
         out["bfs_cu_scaling"] = run_cu_scaling(grus, ctx, g, args)
-    return out
 
 
 def run_pagerank(grus, O, g, h, threads, args, out):

@@ -1,6 +1,8 @@
 """Command-line tools: flag handling that needs no GPU."""
+import json
 import os
 import subprocess
+import sys
 
 import pytest
 
@@ -112,3 +114,20 @@ def test_traffic_kept_with_window_settings(tmp_path, monkeypatch):
     assert bench.traffic("bfs_w", {"bfs_tuning": [14.0, 64.0, 32]}) == 100 + 5 * 1024
     assert bench.traffic("bfs_w", {"bfs_tuning": [14.0, 24.0, 32]}) is None
     assert bench.traffic("bfs_w") == 100 + 5 * 1024
+
+
+def test_bench_watchdog_reports_without_a_stuck_leg():
+    # a leg that never returns is abandoned: the line is printed with that
+    # leg marked, then the process exits on its own
+    code = ("import sys, time; sys.path.insert(0, %r); import bench; "
+            "w = bench.Watchdog(0.5); "
+            "w.line = {'value': 1.0, 'extra': {'pagerank': {'x': 1, 'roofline': {'f': 2}}}}; "
+            "w.leg('stuck'); time.sleep(30); print('not reached')" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = [l for l in r.stdout.splitlines() if l.strip()]
+    assert len(out) == 1 and "not reached" not in r.stdout
+    line = json.loads(out[0])
+    assert line["extra"]["stuck"]["error"].startswith("no result within")
+    assert line["roofline_pagerank"] == {"f": 2} and line["extra"]["pagerank"] == {"x": 1}
+    assert "leg stuck" in r.stderr
