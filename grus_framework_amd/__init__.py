@@ -317,7 +317,8 @@ class Server:
     """Continuous-batching query server (grus_server_*)."""
 
     def __init__(self, g, max_inflight_batches=0, max_batch=0, state_budget_bytes=0,
-                 copy_results=False, admit_min_free=0, admit_window=0, solo_max=0):
+                 copy_results=False, admit_min_free=0, admit_window=0, solo_max=0,
+                 idle_release_ms=0):
         """Zero for max_inflight_batches, max_batch, admit_min_free or
         admit_window picks the library default (4 engines per kind, 64 lanes,
         a batch admits arrivals during its first 2 steps, later only when
@@ -325,10 +326,14 @@ class Server:
         single-source kernels while that would empty the queue within one
         batch's time); solo_max > 0 fixes that queue length instead,
         admit_window < 0 turns the window off, solo_max < 0 the single-query
-        path."""
+        path.  state_budget_bytes caps the device bytes of all engines' state:
+        engines beyond the first of each kind take state only when it fits,
+        and give it back after idle_release_ms (0 = 1000 under a cap or a
+        device budget, else never; < 0 = never)."""
         self.g = g
         opts = capi.ServerOpts(max_inflight_batches, max_batch, state_budget_bytes,
-                               int(copy_results), admit_min_free, admit_window, solo_max)
+                               int(copy_results), admit_min_free, admit_window, solo_max,
+                               idle_release_ms)
         h = C.c_void_p()
         capi.check(_lib().grus_server_create(g.h, C.byref(opts), C.byref(h)))
         self.h = h
@@ -352,6 +357,14 @@ class Server:
         if out is not None:
             self._outs[qid.value] = out
         return qid.value
+
+    def stats(self):
+        """Memory-based admission counters (grus_server_stats)."""
+        st = capi.ServerStats()
+        capi.check(_lib().grus_server_stats_get(self.h, C.byref(st)))
+        return {"engines_held": (st.engines_held[0], st.engines_held[1]),
+                "state_bytes": st.state_bytes, "engines_started": st.engines_started,
+                "engines_released": st.engines_released, "memory_waits": st.memory_waits}
 
     def poll(self, timeout_ms=-1):
         r = capi.QueryResult()

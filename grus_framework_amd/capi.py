@@ -63,7 +63,14 @@ class CsrViewC(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("max_inflight_batches", C.c_int), ("max_batch", C.c_int),
                 ("state_budget_bytes", C.c_uint64), ("copy_results", C.c_int),
-                ("admit_min_free", C.c_int), ("admit_window", C.c_int), ("solo_max", C.c_int)]
+                ("admit_min_free", C.c_int), ("admit_window", C.c_int), ("solo_max", C.c_int),
+                ("idle_release_ms", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("engines_held", C.c_int * 2), ("state_bytes", C.c_uint64),
+                ("engines_started", C.c_uint64), ("engines_released", C.c_uint64),
+                ("memory_waits", C.c_uint64)]
 
 
 class QueryResult(C.Structure):
@@ -118,6 +125,7 @@ SIGNATURES = [
     ("grus_server_submit", _i, [_vp, _i, _u32, _vp, C.POINTER(_u64)]),
     ("grus_server_poll", _i, [_vp, C.POINTER(QueryResult), _i]),
     ("grus_server_destroy", _i, [_vp]),
+    ("grus_server_stats_get", _i, [_vp, C.POINTER(ServerStats)]),
 ]
 
 _LIB = None

@@ -441,4 +441,12 @@ int grus_server_destroy(grus_server *s) {
   return guard([&] { delete s; });
 }
 
+int grus_server_stats_get(grus_server *s, grus_server_stats *out) {
+  return guard([&] {
+    need(s, "s");
+    need(out, "out");
+    s->s->stats(out);
+  });
+}
+
 }  // extern "C"

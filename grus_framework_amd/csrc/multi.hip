@@ -682,7 +682,18 @@ MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream)
   } else {
     ctx_ = g.ctx;
   }
+  init();
+}
+
+MsEngine::MsEngine(Graph &g, MsKind kind, int lanes, Context *ctx)
+    : g_(g), kind_(kind), lanes_(std::max(1, std::min(lanes, 64))), ctx_(ctx) {
+  slots_ = kind_ == MsKind::BFS ? lanes_ : 2 * lanes_;
+  init();
+}
+
+void MsEngine::init() {
   Context &ctx = *ctx_;
+  Graph &g = g_;
   ctx.activate();
   const uint64_t V = g.V;
   if (kind_ == MsKind::BFS) visit_.alloc((V + 1) * 8);

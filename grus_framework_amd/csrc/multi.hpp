@@ -59,8 +59,11 @@ inline int slot_lowest(SlotMask m) {
 
 class MsEngine {
  public:
-  // lanes: 1..64 (BFS: one slot each; SSSP: two slots each)
+  // lanes: 1..64 (BFS: one slot each; SSSP: two slots each).  own_stream:
+  // a context (stream) of its own, else the graph's
   MsEngine(Graph &g, MsKind kind, int lanes, bool own_stream);
+  // the same on a context the caller owns (it must outlive the engine)
+  MsEngine(Graph &g, MsKind kind, int lanes, Context *ctx);
   ~MsEngine();
   MsKind kind() const { return kind_; }
   int slots() const { return slots_; }
@@ -111,6 +114,7 @@ class MsEngine {
   void widen(SlotMask restart);
   MsCounters publish();
   uint64_t read_u64(const uint64_t *d);
+  void init();  // device state, on ctx_
 };
 
 }  // namespace mgg

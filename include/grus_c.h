@@ -344,7 +344,16 @@ typedef struct grus_server_opts {
                                  fewer engines give lower tail latency) */
   int max_batch;              /* lanes per batch, 1..64, default 64; a BFS
                                  lane holds one query, an SSSP lane two */
-  uint64_t state_budget_bytes; /* device bytes for per-query state (0 = auto) */
+  uint64_t state_budget_bytes; /* device bytes all engines' per-query state
+                                 may hold together (0 = no cap).  Engines take
+                                 their state when a batch needs one and it
+                                 fits under this cap and under the device
+                                 budget; until then waiting queries stay
+                                 queued.  The first engine of each kind holds
+                                 its state from creation (lanes shrink when
+                                 one engine per kind at max_batch lanes would
+                                 not fit), the others give theirs back after
+                                 idle_release_ms without a query */
   int copy_results;           /* 1: copy each result to the host buffer given
                                  at submit; 0: keep a digest only */
   int admit_min_free;         /* past its admission window, an engine that
@@ -371,7 +380,21 @@ typedef struct grus_server_opts {
                                  single queries until one has run; at most
                                  8 waiting until a single query has run);
                                  negative = never */
+  int idle_release_ms;        /* an engine other than the first of its kind
+                                 that has run no query for this long gives
+                                 its device state back.  0 = default: 1000
+                                 when state_budget_bytes or a device budget
+                                 caps memory, else never; negative = never */
 } grus_server_opts;
+/* Memory-based admission, as counted by the server (no reference
+ * counterpart). */
+typedef struct grus_server_stats {
+  int engines_held[2];      /* engines holding device state, per kind */
+  uint64_t state_bytes;     /* the device bytes of that state */
+  uint64_t engines_started; /* engines that took state (including the first) */
+  uint64_t engines_released;
+  uint64_t memory_waits;    /* times a batch found no room for an engine */
+} grus_server_stats;
 typedef struct grus_query_result {
   uint64_t id;
   int status;            /* GRUS_OK or error */
@@ -386,6 +409,7 @@ int grus_server_submit(grus_server *s, int kind, uint32_t src, uint32_t *out_hos
 /* Wait up to timeout_ms (<0 = forever) for one finished query. */
 int grus_server_poll(grus_server *s, grus_query_result *res, int timeout_ms);
 int grus_server_destroy(grus_server *s);
+int grus_server_stats_get(grus_server *s, grus_server_stats *out);
 
 #ifdef __cplusplus
 }

@@ -206,6 +206,86 @@ def test_server_alone_equals_in_mix(ctx):
         assert (mixed.reached, mixed.checksum) == (alone.reached, alone.checksum)
 
 
+def _mixed_queries(g, n, seed):
+    h = host_csr(g)
+    rng = np.random.default_rng(seed)
+    nz = np.nonzero(h.degrees())[0]
+    return [(int(rng.integers(0, 2)), int(rng.choice(nz))) for _ in range(n)]
+
+
+def _serve_all(srv, qs, cap=None):
+    """Submits every query at once, polls them all; returns the results by
+    query and the largest engine count seen (checking the state cap)."""
+    ids = {srv.submit(kind, s): (kind, s) for kind, s in qs}
+    got, most = {}, 0
+    for _ in range(len(qs)):
+        r = srv.poll(60000)
+        assert r.status == 0
+        got[r.id] = r
+        st = srv.stats()
+        if cap is not None:
+            assert st["state_bytes"] <= cap, st
+        most = max(most, sum(st["engines_held"]))
+    assert set(got) == set(ids)
+    return {ids[q]: (r.reached, r.checksum) for q, r in got.items()}, most
+
+
+def test_server_admits_by_state_memory(ctx):
+    # engines beyond the first of each kind take device state only while it
+    # fits the state budget; the rest of a burst waits for it, and an idle
+    # extra engine gives its state back
+    import time
+    with grus.Graph.rmat(ctx, 13, weights=(1, 255), wseed=3) as g:
+        qs = _mixed_queries(g, 300, 17)
+        ref = {grus.Q_BFS: single_digests(g, grus.Q_BFS, [s for k, s in qs if k == 0]),
+               grus.Q_SSSP: single_digests(g, grus.Q_SSSP, [s for k, s in qs if k == 1])}
+        with grus.Server(g, solo_max=-1) as probe:
+            st = probe.stats()
+        assert st["engines_held"] == (1, 1) and st["engines_started"] == 2
+        both = st["state_bytes"]  # one engine of each kind, about the same size
+        cap = both * 8 // 5  # room for one more engine, not two
+        with grus.Server(g, state_budget_bytes=cap, idle_release_ms=100, solo_max=-1) as srv:
+            res, most = _serve_all(srv, qs, cap)
+            st = srv.stats()
+            assert most == 3, most
+            assert st["memory_waits"] > 0 and st["engines_started"] >= 3, st
+            time.sleep(0.6)  # past the idle time: the extra engine gives its state back
+            st = srv.stats()
+            assert st["engines_held"] == (1, 1) and st["engines_released"] >= 1, st
+            assert st["state_bytes"] == both
+            # state is taken again for the next burst
+            res2, _ = _serve_all(srv, qs[:120], cap)
+        for (kind, s), d in list(res.items()) + list(res2.items()):
+            assert d == ref[kind][s], (kind, s)
+
+
+def test_server_admits_by_device_budget(ctx):
+    # the same with the device budget as the limit: an extra engine's
+    # allocation is refused with GRUS_E_BUDGET and its batch waits
+    with grus.Graph.rmat(ctx, 13, weights=(1, 255), wseed=3) as g:
+        qs = _mixed_queries(g, 200, 23)
+        ref = {grus.Q_BFS: single_digests(g, grus.Q_BFS, [s for k, s in qs if k == 0]),
+               grus.Q_SSSP: single_digests(g, grus.Q_SSSP, [s for k, s in qs if k == 1])}
+        held0, _ = ctx.device_memory()
+        with grus.Server(g, solo_max=-1) as probe:
+            held1, _ = ctx.device_memory()
+            both = probe.stats()["state_bytes"]
+        budget = held1 + both // 4  # no room for another engine
+        try:
+            ctx.set_device_budget(budget)
+            ctx.device_memory(reset_peak=True)
+            with grus.Server(g, solo_max=-1) as srv:
+                res, most = _serve_all(srv, qs)
+                st = srv.stats()
+            assert most == 2 and st["memory_waits"] > 0, st
+            assert ctx.device_memory()[1] <= budget
+        finally:
+            ctx.set_device_budget(0)
+        assert held1 > held0
+        for (kind, s), d in res.items():
+            assert d == ref[kind][s], (kind, s)
+
+
 def test_server_errors(ctx):
     with grus.Graph.rmat(ctx, 10) as g:
         with pytest.raises(grus.GrusError) as e:
