@@ -539,15 +539,16 @@ def run_serving(grus):
                 srv.submit(int(kinds[i]), int(srcs[i]))
             out = [srv.poll(300000) for _ in range(N)]
             tb = time.perf_counter() - t0
+            st = srv.stats()
         assert all(r.status == 0 for r in out)
-        return N / tb
+        return N / tb, st
 
     # the library's default policy; and lane-by-lane admission into 8
     # engines per kind
     REGIONS.begin("serving_rmat_s22_saturation_query")
-    qps = saturate()
+    qps, sat_stats = saturate()
     REGIONS.end(N)
-    qps_cont = saturate(max_inflight_batches=8, admit_min_free=1)
+    qps_cont, _ = saturate(max_inflight_batches=8, admit_min_free=1)
     res["batched_qps"] = round(qps, 1)
     res["speedup_vs_serial"] = round(qps / serial_qps, 2)
     res["policy"] = ("4 engines per kind x 64 lanes (BFS 64 queries, SSSP 128); an idle engine "
@@ -556,6 +557,9 @@ def run_serving(grus):
                      "engine serves its kind's queue one query at a time with the single-source "
                      "kernels while that empties the queue within one batch time")
     res["continuous_8_engines_qps"] = round(qps_cont, 1)
+    # engines take device state as batches need it (admission by memory)
+    res["saturation_engines_started"] = int(sat_stats["engines_started"])
+    res["saturation_state_GB"] = round(sat_stats["state_bytes"] / 1e9, 2)
     # one full SSSP batch alone (128 queries in 64 lanes) and one of 64:
     # the reference points for latency
     for nq in (64, 128):
@@ -581,11 +585,13 @@ def run_serving(grus):
                     pass
                 srv.submit(int(kinds[i]), int(srcs[i]))
             out = [srv.poll(300000) for _ in range(NP)]
+            st = srv.stats()
         assert all(r.status == 0 for r in out)
         lat = np.array([(r.done_us - r.submit_us) / 1e3 for r in out])
         sweep.append({"load": frac, "rate_qps": round(rate, 1),
                       "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                      "p99_ms": round(float(np.percentile(lat, 99)), 2)})
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                      "engines_started": int(st["engines_started"])})
         total += NP
     mid = sweep[1]
     res.update({"poisson_rate_qps": mid["rate_qps"], "p50_ms": mid["p50_ms"],
