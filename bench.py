@@ -622,17 +622,25 @@ def run_capped(grus, ctx):
             held, _ = ctx.device_memory(reset_peak=True)
             src = g.info.max_degree_vertex
             grus.bfs(g, src, copy=False)
+            REGIONS.begin("bfs_rmat_s27_%s" % mode)
             _, st = grus.bfs(g, src, copy=False)
+            REGIONS.end(1)
             digest = g.digest()
             # top-down only: rows read where they live (zero-copy over the
             # link when capped), and the two-stage streaming BFS (rows
             # gathered into a device buffer per level, then expanded)
+            REGIONS.begin("bfs_push_rmat_s27_%s" % mode)
             _, sp_push = grus.bfs(g, src, "push", copy=False)
+            REGIONS.end(1)
             push_digest = g.digest()
             grus.bfs_2s(g, src, copy=False)
+            REGIONS.begin("bfs_2s_rmat_s27_%s" % mode)
             _, s2 = grus.bfs_2s(g, src, copy=False)
+            REGIONS.end(1)
             s2_digest = g.digest()
+            REGIONS.begin("pagerank_2it_rmat_s27_%s" % mode)
             _, sp = grus.pagerank(g, 0.85, 2, copy=False)
+            REGIONS.end(1)
             _, peak = ctx.device_memory()
             B = 2 * (8 * (g.V + 1) + 8 * g.E + 12 * g.V)
             res[mode] = {"bfs_ms": round(st.device_ms, 2),
