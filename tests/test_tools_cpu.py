@@ -131,3 +131,28 @@ def test_bench_watchdog_reports_without_a_stuck_leg():
     assert line["extra"]["stuck"]["error"].startswith("no result within")
     assert line["roofline_pagerank"] == {"f": 2} and line["extra"]["pagerank"] == {"x": 1}
     assert "leg stuck" in r.stderr
+
+
+SWEEP = os.path.join(ROOT, "grus_framework_amd", "cli", "run_sweep.sh")
+
+
+def test_sweep_script_commands(tmp_path):
+    # the reference's dataset list and sources by default (run.sh:9-16)
+    r = subprocess.run(["bash", SWEEP, "--dry-run", "--data", "/d"], capture_output=True,
+                       text=True, timeout=30)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split("\n")[:-1]
+    assert len(lines) == 7
+    assert lines[4].endswith("grus_bfs --input /d/sk-2005.w.gr --src 10")
+    assert lines[5].endswith("--input /d/friendster.w.gr --src 101")
+    # own pairs, another app, extra flags; pr and cc take no source
+    r = subprocess.run(["bash", SWEEP, "--dry-run", "--app", "cc", "--suffix", ".gr", "g1:3",
+                        "--", "--method", "lp"], capture_output=True, text=True, timeout=30)
+    assert r.stdout.strip().endswith("grus_cc --input %s/data/g1.gr --method lp"
+                                     % os.path.expanduser("~"))
+    # a missing input is skipped, not an error; an unknown app is
+    r = subprocess.run(["bash", SWEEP, "--data", str(tmp_path), "x:0"], capture_output=True,
+                       text=True, timeout=30)
+    assert r.returncode == 0 and "not found, skipped" in r.stderr
+    r = subprocess.run(["bash", SWEEP, "--app", "tc"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 64

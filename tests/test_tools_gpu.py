@@ -54,6 +54,20 @@ def test_cli_dump_matches_oracle(tmp_path, oracle, app):
         assert (load_dump(out, np.uint64).astype(np.uint32) == ref).all()
 
 
+def test_sweep_runs_each_dataset(tmp_path, oracle):
+    # run_sweep.sh (the reference's run.sh) over two .w.gr files and one
+    # missing name: one run per present file, each with its own source
+    for name, scale in (("a", 9), ("b", 10)):
+        g = oracle.gen_rmat(scale, weights=(1, 255), wseed=3)
+        grus.gr_write(str(tmp_path / (name + ".w.gr")), g.off, g.col, g.w)
+    r = subprocess.run(["bash", os.path.join(ROOT, "grus_framework_amd", "cli", "run_sweep.sh"),
+                        "--data", str(tmp_path), "--app", "sssp", "a:3", "nope:0", "b:7",
+                        "--", "--symmetric"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("itr ") == 2 and "== a (src 3)" in r.stdout
+    assert "== b (src 7)" in r.stdout and "nope.w.gr not found" in r.stderr
+
+
 def test_cli_placement_flags(tmp_path, oracle):
     out = str(tmp_path / "o.txt")
     for extra in (["--noopt", "--pf", "--ab"], ["--placement", "capped", "--hbm_cap_gb", "0.001"],
