@@ -24,12 +24,13 @@
 // queue of 8 already starts a batch (which then slows the single queries
 // beside it).
 //
-// Memory.  Each worker owns a context (stream); its engine's device state
-// (the per-vertex rows of its slots) is taken when a batch needs it and
-// fits under state_budget_bytes and the device budget, and given back after
-// idle_release_ms without a query; queries wait while there is no room.
-// The first engine of each kind holds its state from creation, so every
-// kind always makes progress.
+// Memory.  Each worker owns a context (stream).  When memory is capped
+// (state_budget_bytes or a device budget), its engine's device state (the
+// per-vertex rows of its slots) is taken when a batch needs it and fits
+// under both, and given back after idle_release_ms without a query;
+// queries wait while there is no room.  The first engine of each kind
+// holds its state from creation, so every kind always makes progress.
+// Uncapped, every engine holds its state from creation.
 //
 // Admission.  An idle engine takes up to a full batch.  A running engine
 // takes arrivals into its free lanes during the first admit_window steps
@@ -129,10 +130,14 @@ class ServerImpl : public Server {
     }
     engines_.resize(n);
     state_.assign(n, 0);
-    // the first engine of each kind holds its state from the start
-    for (int k = 0; k < 2; k++)
-      if (!start_engine((size_t)(k * per_kind_)))
+    // the first engine of each kind holds its state from the start; with
+    // memory uncapped every engine does (nothing to admit by, and no
+    // allocation in a worker thread while the others launch)
+    for (int i = 0; i < n; i++) {
+      if (capped && i % per_kind_ != 0) continue;
+      if (!start_engine((size_t)i))
         raise(GRUS_E_BUDGET, "no device room for one engine of each query kind");
+    }
     // each worker's view of the graph for single-source queries; an SSSP
     // view builds its pull-round layout now, not in its first query
     if (solo_max_ > 0) {

@@ -239,7 +239,7 @@ def test_server_admits_by_state_memory(ctx):
         qs = _mixed_queries(g, 300, 17)
         ref = {grus.Q_BFS: single_digests(g, grus.Q_BFS, [s for k, s in qs if k == 0]),
                grus.Q_SSSP: single_digests(g, grus.Q_SSSP, [s for k, s in qs if k == 1])}
-        with grus.Server(g, solo_max=-1) as probe:
+        with grus.Server(g, solo_max=-1, max_inflight_batches=1) as probe:
             st = probe.stats()
         assert st["engines_held"] == (1, 1) and st["engines_started"] == 2
         both = st["state_bytes"]  # one engine of each kind, about the same size
@@ -267,7 +267,7 @@ def test_server_admits_by_device_budget(ctx):
         ref = {grus.Q_BFS: single_digests(g, grus.Q_BFS, [s for k, s in qs if k == 0]),
                grus.Q_SSSP: single_digests(g, grus.Q_SSSP, [s for k, s in qs if k == 1])}
         held0, _ = ctx.device_memory()
-        with grus.Server(g, solo_max=-1) as probe:
+        with grus.Server(g, solo_max=-1, max_inflight_batches=1) as probe:
             held1, _ = ctx.device_memory()
             both = probe.stats()["state_bytes"]
         budget = held1 + both // 4  # no room for another engine
@@ -284,6 +284,15 @@ def test_server_admits_by_device_budget(ctx):
         assert held1 > held0
         for (kind, s), d in res.items():
             assert d == ref[kind][s], (kind, s)
+
+
+def test_server_uncapped_engines_hold_state_from_creation(ctx):
+    with grus.Graph.rmat(ctx, 11) as g:
+        with grus.Server(g, max_inflight_batches=3, solo_max=-1) as srv:
+            st = srv.stats()
+            assert st["engines_held"] == (3, 3) and st["engines_started"] == 6
+            res, most = _serve_all(srv, _mixed_queries(g, 40, 3))
+            assert most == 6 and srv.stats()["engines_released"] == 0
 
 
 def test_server_errors(ctx):
