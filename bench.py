@@ -217,6 +217,8 @@ def main():
                     help="first run a 1 GiB device-to-device copy in its own timed window")
     ap.add_argument("--regions-out", default="",
                     help="write the host-clock windows of the timed loops to this JSON file")
+    ap.add_argument("--leg-child", default="", choices=["", "serving", "cu", "capped"],
+                    help=argparse.SUPPRESS)  # one leg, run as a child process
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -225,6 +227,24 @@ def main():
 
     import grus_framework_amd as grus
     from oracle import oracle_py as O
+
+    if args.leg_child:
+        if args.leg_child == "serving":
+            res = run_serving(grus)
+        else:
+            cctx = grus.Context(0)
+            if args.leg_child == "cu":
+                with grus.Graph.rmat(cctx, args.scale) as cg:
+                    res = run_cu_scaling(grus, cctx, cg, args)
+            else:
+                try:
+                    res = run_capped(grus, cctx)
+                except grus.GrusError as e:  # e.g. not enough device or host memory
+                    res = {"error": str(e)}
+            cctx.close()
+        REGIONS.write(args.regions_out)
+        print(json.dumps(res), flush=True)
+        return
 
     if args.copy_calibration:
         copy_calibration()
@@ -337,10 +357,7 @@ def main():
     only = set(x for x in args.only.split(",") if x)
     if not args.no_extra and (not only or "capped" in only):
         watch.leg("capped_rmat_s27")
-        try:
-            extra["capped_rmat_s27"] = run_capped(grus, ctx)
-        except grus.GrusError as e:  # e.g. not enough device or host memory
-            extra["capped_rmat_s27"] = {"error": str(e)}
+        extra["capped_rmat_s27"] = run_leg_child("capped", args)
     watch.name = None  # every leg done
     finish_line(line)
     watch.emit(line)
@@ -366,10 +383,10 @@ def run_extra(grus, O, ctx, g, h, threads, args, out, watch):
                  rmat=want("sssp") or want("sssp_rmat"), grid=want("sssp") or want("sssp_grid"))
     if want("serving"):
         watch.leg("serving_rmat_s22")
-        out["serving_rmat_s22"] = run_serving(grus)
+        out["serving_rmat_s22"] = run_leg_child("serving", args)
     if want("cu"):
         watch.leg("bfs_cu_scaling")
-        out["bfs_cu_scaling"] = run_cu_scaling(grus, ctx, g, args)
+        out["bfs_cu_scaling"] = run_leg_child("cu", args)
 
 
 def run_pagerank(grus, O, g, h, threads, args, out):
@@ -495,6 +512,32 @@ def run_cu_scaling(grus, ctx, g, args):
         ctx.set_cu_mask(None)
     log("bfs GTEPS by CU count: %s" % res)
     return res
+
+
+def run_leg_child(name, args):
+    """One extra leg (serving, cu, capped) in a child process with its own
+    context and graph: the leg's worker threads, streams, CU-masked queues
+    and device budget stay out of this process, and a crash there costs the
+    leg, not the line.  Its log lines and timed windows are passed on."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        regions = os.path.join(tmp, "regions.json")
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--leg-child", name,
+               "--scale", str(args.scale), "--regions-out", regions]
+        limit = args.leg_limit - 10 if args.leg_limit > 20 else None
+        try:
+            # its log lines go straight to this process's stderr
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=limit)
+        except subprocess.TimeoutExpired:
+            return {"error": "%s leg gave no result within %d s" % (name, limit)}
+        if os.path.exists(regions):
+            with open(regions) as f:
+                REGIONS.windows.extend(json.load(f))
+    lines = [l for l in r.stdout.splitlines() if l.strip()]
+    if r.returncode != 0 or not lines:
+        return {"error": "%s leg exited with status %d" % (name, r.returncode)}
+    return json.loads(lines[-1])
 
 
 def run_serving(grus):

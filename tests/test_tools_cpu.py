@@ -156,3 +156,14 @@ def test_sweep_script_commands(tmp_path):
     assert r.returncode == 0 and "not found, skipped" in r.stderr
     r = subprocess.run(["bash", SWEEP, "--app", "tc"], capture_output=True, text=True, timeout=30)
     assert r.returncode == 64
+
+
+def test_bench_serving_leg_failure_is_reported():
+    # the serving leg runs in a child process; when it cannot run (here: no
+    # GPU) the leg reports an error instead of taking the line down
+    import argparse
+    sys.path.insert(0, ROOT)
+    import bench
+    for leg in ("serving", "cu", "capped"):
+        res = bench.run_leg_child(leg, argparse.Namespace(leg_limit=120.0, scale=10))
+        assert set(res) == {"error"} and ("%s leg exited with status" % leg) in res["error"]
