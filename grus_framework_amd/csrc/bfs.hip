@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
+#include <cstring>
 
 #include "advance.hpp"
 #include "apps.hpp"
@@ -303,6 +304,119 @@ constexpr uint64_t SPARSE_DIV = 16;
 // (RMAT s24, pinned thresholds: V/16 580, V/32 584, V/64 583, V/128 582
 // GTEPS; 551 without stamps)
 constexpr uint64_t STAMP_DIV = 64;
+// Several small top-down levels in one launch of one workgroup: while the
+// frontier has at most SL_F vertices and SL_E out-edges (and the direction
+// rule keeps pushing), the workgroup expands a level, claims the new
+// vertices in the visited bitmap, writes their levels, appends them to the
+// other queue and goes on, with workgroup barriers between levels instead
+// of a return to the host.  The first and last levels of most traversals
+// (a source of low degree, the tail of the reached component) are such
+// levels; each costs a host round trip of ~20 us when launched alone.
+constexpr uint32_t SL_T = 1024;                  // threads of the workgroup
+constexpr uint32_t SL_PER = 4;                   // edges per thread per level
+constexpr uint32_t SL_F = SL_T;                  // frontier vertices staged in LDS
+constexpr uint64_t SL_E = (uint64_t)SL_T * SL_PER;
+constexpr uint32_t SL_LEVELS = 64;               // levels per launch at most
+
+// what the launch leaves behind (in the control block, after the counters)
+struct SmallLevels {
+  uint64_t mf, mu, v_add, e_add;  // frontier edges, unexplored edges, reached sums
+  uint32_t nf, levels, cur, prev_nf;
+};
+
+__global__ __launch_bounds__(SL_T) void k_small_levels(CsrView g, const uint64_t *out_off,
+                                                       uint32_t *q0, uint32_t *q1, int cur0,
+                                                       uint32_t nf0, uint64_t mf0, uint64_t mu0,
+                                                       double alpha, uint32_t *label,
+                                                       uint64_t *visited, uint32_t level0,
+                                                       SmallLevels *out) {
+  __shared__ uint64_t sS[SL_F + 1];
+  __shared__ uint64_t sRow[SL_F];
+  __shared__ uint64_t s_wsum[SL_T / WAVE];
+  __shared__ uint32_t s_nn;
+  __shared__ unsigned long long s_mn;
+  uint32_t *q[2] = {q0, q1};
+  const uint32_t t = threadIdx.x, lane = lane_id(), wid = t / WAVE;
+  int cur = cur0;
+  uint32_t nf = nf0, prev = 0, lv = 0;
+  uint64_t mf = mf0, mu = mu0, vadd = 0, eadd = 0;
+  // every thread holds the same nf / mf / mu / lv, so the loop is uniform.
+  // The host chose push for the first level (its pull -> push switch does
+  // not test alpha); later levels follow the host's push -> pull rule
+  while (nf > 0 && lv < SL_LEVELS && nf <= SL_F && mf <= SL_E &&
+         !(lv > 0 && alpha > 0 && (double)mf > (double)mu / alpha)) {
+    // stage the frontier (thread t owns entry t) and scan its degrees
+    uint64_t d = 0;
+    if (t < nf) {
+      const uint32_t u = q[cur][t];
+      const uint64_t a = g.off[u];
+      sRow[t] = a;
+      d = g.off[u + 1] - a;
+    }
+    uint64_t x = d;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o);
+      if ((int)lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) s_wsum[wid] = x;
+    __syncthreads();
+    uint64_t before = 0, sum = 0;
+    for (uint32_t w2 = 0; w2 < SL_T / WAVE; w2++) {
+      before += (w2 < wid) ? s_wsum[w2] : 0ull;
+      sum += s_wsum[w2];
+    }
+    if (t < nf) sS[t] = before + x - d;
+    if (t == 0) {
+      sS[nf] = sum;
+      s_nn = 0;
+      s_mn = 0;
+    }
+    __syncthreads();
+    // the level's edges, SL_PER per thread: columns first, then the claims
+    uint32_t dst[SL_PER];
+#pragma unroll
+    for (uint32_t r = 0; r < SL_PER; r++) {
+      const uint64_t k = t + (uint64_t)r * SL_T;
+      dst[r] = GRUS_NO_VERTEX;
+      if (k < sum) {
+        const uint32_t j = upper_bound_u64(sS, 0, nf + 1, k) - 1;
+        dst[r] = edge_col(g, sRow[j] + (k - sS[j]));
+      }
+    }
+    uint64_t my_e = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < SL_PER; r++) {
+      const uint32_t v = dst[r];
+      if (v == GRUS_NO_VERTEX || bm_test(visited, v) || !bm_claim(visited, v)) continue;
+      label[v] = level0 + lv + 1;
+      q[cur ^ 1][atomicAdd(&s_nn, 1u)] = v;
+      my_e += out_off[v + 1] - out_off[v];
+    }
+    if (my_e) atomicAdd(&s_mn, (unsigned long long)my_e);
+    __syncthreads();
+    // (s_nn / s_mn are reset only after the next level's first barrier)
+    prev = nf;
+    nf = s_nn;
+    mf = s_mn;
+    mu = mu >= mf ? mu - mf : 0;
+    vadd += nf;
+    eadd += mf;
+    cur ^= 1;
+    lv++;
+  }
+  if (t == 0) {
+    out->mf = mf;
+    out->mu = mu;
+    out->v_add = vadd;
+    out->e_add = eadd;
+    out->nf = nf;
+    out->levels = lv;
+    out->cur = (uint32_t)cur;
+    out->prev_nf = prev;
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_pull_sparse(
     CsrView in, const uint64_t *out_off, const uint32_t *first_in, const uint64_t *front,
     uint64_t *visited, uint64_t *front_next, uint32_t *label, uint32_t next_level, uint32_t K,
@@ -526,6 +640,38 @@ void run_bfs(Graph &g, uint32_t src, int dir, grus_stats *st) {
                         &cnt->long_count);  // scratch counter, re-zeroed below
         GRUS_HIP(hipMemsetAsync(cnt, 0, sizeof(StepCounters), ctx.stream));
         launches += 1;
+      }
+      if (nf <= bfs::SL_F && mf <= bfs::SL_E) {
+        // small levels: one workgroup runs them back to back on the device
+        auto *sl = reinterpret_cast<bfs::SmallLevels *>(ctx.d_ctl.as<uint64_t>() + 8);
+        bfs::k_small_levels<<<1, bfs::SL_T, 0, ctx.stream>>>(
+            out, out.off, w.q[0].as<uint32_t>(), w.q[1].as<uint32_t>(), cur, (uint32_t)nf, mf,
+            mu, dir == GRUS_DIR_AUTO ? alpha : 0.0, label, w.visited.as<uint64_t>(), level, sl);
+        GRUS_LAUNCH_CHECK();
+        launches++;
+        ctx.publish_ctl(8 + sizeof(bfs::SmallLevels) / 8);
+        bfs::SmallLevels r;
+        std::memcpy(&r, ctx.h_ctl + 8, sizeof r);
+        if (r.nf > V || r.levels == 0 || r.cur > 1)
+          raise(GRUS_E_HIP, "bfs small levels reported an impossible state");
+        if (trace) {
+          auto now = std::chrono::steady_clock::now();
+          fprintf(stderr, "bfs levels %u-%u in one launch: nf=%llu -> %u %.1f us\n", level,
+                  level + r.levels - 1, (unsigned long long)nf, r.nf,
+                  std::chrono::duration<double, std::micro>(now - t_level).count());
+          t_level = now;
+        }
+        level += r.levels;
+        cur = (int)r.cur;
+        prev_nf = r.prev_nf;
+        nf = r.nf;
+        mf = r.mf;
+        mu = r.mu;
+        v_reached += r.v_add;
+        e_reached += r.e_add;
+        have_queue = true;  // the frontier is queue cur; no bitmap of it
+        have_bitmap = false;
+        continue;
       }
       bfs::job_t job{label, w.visited.as<uint64_t>(), out.off, level + 1};
       if (mf * bfs::STAMP_DIV > V) {

@@ -203,6 +203,37 @@ def test_bfs_closed_forms_and_edge_cases(ctx):
         assert lv.tolist() == [0]
 
 
+def test_bfs_small_levels_hand_back(ctx, oracle):
+    # a complete binary tree (frontiers double: levels run in one workgroup
+    # until the frontier outgrows it, then return to the host loop) with a
+    # 300-vertex path hung from its last leaf (back to small levels)
+    n, tail = (1 << 15) - 1, 300
+    child = np.arange(1, n, dtype=np.int64)
+    src = np.concatenate([(child - 1) // 2, child])
+    dst = np.concatenate([child, (child - 1) // 2])
+    p = np.arange(n, n + tail, dtype=np.int64)
+    prev = np.concatenate([[n - 1], p[:-1]])
+    src = np.concatenate([src, prev, p])
+    dst = np.concatenate([dst, p, prev])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    V = n + tail
+    off = np.zeros(V + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    col = dst.astype(np.uint32)
+    h = oracle.CSR(off, col)
+    with grus.Graph.from_csr(ctx, off, col, symmetric=True) as g:
+        for s0 in (0, n - 1, V - 1):
+            want = oracle.bfs(h, s0)
+            for d in ("auto", "push"):
+                lv, st = grus.bfs(g, s0, d)
+                assert (lv == want).all(), (s0, d)
+                assert st.iterations == int(want.max()) + 1
+                assert st.vertices_reached == V
+                assert st.edges_reached == int(off[-1])
+
+
 def test_bfs_repeatable_and_result_copy(ctx):
     with grus.Graph.rmat(ctx, 14) as g:
         s = g.info.max_degree_vertex
