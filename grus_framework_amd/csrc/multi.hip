@@ -722,7 +722,15 @@ void MsEngine::init() {
   fill_u64(ctx, front_.as<uint64_t>(), V + 1, 0);
   fill_u64(ctx, next_.as<uint64_t>(), V + 1, 0);
   GRUS_HIP(hipMemsetAsync(cnt_.p, 0, sizeof(MsCounters), ctx.stream));
-  push_.ensure(V);
+  // a queue holds at most V + 128 vertices (restarted slots' sources may
+  // repeat), and its degree scan one entry more; the scan's temporaries are
+  // the engine's own, so that a step allocates nothing (under a device
+  // budget another engine's start may hold the headroom at that moment)
+  push_.ensure(V + 128);
+  const size_t tb = std::max<size_t>(256, scan_excl_u64_bytes(ctx, V + 129));
+  scan_tmp_.alloc(tb);
+  push_.tmp = scan_tmp_.p;
+  push_.tmp_bytes = tb;
   dirty_ = all_slots();  // state starts uninitialised
   ctx.sync();
 }
@@ -733,7 +741,7 @@ MsEngine::~MsEngine() {
 
 uint64_t MsEngine::state_bytes() const {
   return visit_.bytes + front_.bytes + next_.bytes + vals_.bytes + queue_.bytes + longq_.bytes +
-         row_.bytes;
+         row_.bytes + push_.deg.bytes + push_.scan.bytes + scan_tmp_.bytes;
 }
 
 uint64_t MsEngine::lanes_of(SlotMask m) const {
@@ -922,7 +930,7 @@ std::vector<int> MsEngine::step() {
       ms::k_chunk_counts<<<grid_cap(ctx, qn_ + 1, BLOCK), BLOCK, 0, ctx.stream>>>(
           out.off, queue_.as<uint32_t>(), (uint32_t)qn_, push_.deg_ptr());
       GRUS_LAUNCH_CHECK();
-      scan_excl_u64(ctx, push_.deg_ptr(), cs, qn_ + 1);
+      scan_excl_u64_with(ctx, push_.deg_ptr(), cs, qn_ + 1, push_.tmp, push_.tmp_bytes);
       // no read-back of the chunk count: at most one chunk per queued vertex
       // plus one per CHUNK of its edges (front_edges_ misses the degrees of
       // sources admitted since the last step, which only sizes the grid)

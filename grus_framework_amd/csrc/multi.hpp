@@ -104,6 +104,7 @@ class MsEngine {
   Buf visit_, front_, next_, vals_, queue_, longq_, row_, digest_, srcs_dev_, cnt_, push_cnt_;
   Buf hmirror_;  // pinned host words: the counters as published, then read-back scratch
   PushWork push_;
+  Buf scan_tmp_;  // the step's scan temporaries, owned: a step allocates nothing
   SlotMask all_slots() const {
     return slots_ == 128 ? ~(SlotMask)0 : (slot_bit(slots_) - 1);
   }

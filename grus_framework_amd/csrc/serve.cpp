@@ -99,11 +99,12 @@ class ServerImpl : public Server {
     if (opts_.max_inflight_batches > 8) raise(GRUS_E_INVALID, "at most 8 engines per kind");
     // lanes per engine: one engine of each kind must fit the state budget
     // (a BFS lane holds a V-entry level row, an SSSP lane a V-entry column
-    // of two 16-bit distances; both have V-entry masks and queues beside)
+    // of two 16-bit distances; both have V-entry masks, queues and a degree
+    // scan beside, at most 52 bytes per vertex)
     int lanes = opts_.max_batch;
     if (opts_.state_budget_bytes) {
       uint64_t per_lane = g.V * 4;
-      uint64_t shared = g.V * 8 * 5;
+      uint64_t shared = g.V * 52;
       uint64_t per_engine = opts_.state_budget_bytes / 2;
       if (per_engine <= shared + per_lane)
         raise(GRUS_E_BUDGET, "state budget too small for one lane per engine of each kind");
