@@ -360,6 +360,7 @@ __global__ __launch_bounds__(BLOCK) void k_pad_rows(CsrView g, uint32_t D, uint2
 __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
   __shared__ uint32_t s_near[PQ_CAP], s_far[PQ_CAP], s_loc[PL_CAP], s_locd[PL_CAP];
   __shared__ uint32_t s_nn, s_nf, s_base, s_hi, s_nl;
+  __shared__ uint32_t s_scr[BLOCK / WAVE + 1];
   const CsrView g = p.g;
   unsigned int gen = 0;
   uint32_t step = 0, stamp = p.stamp0;
@@ -404,6 +405,19 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
           stage(v, s_near, &s_nn, &C[0], q[cur ^ 1], 0);
         }
       };
+      // A vertex improved to T or beyond belongs to the far list, once per
+      // far epoch (stamps).  It is kept in LDS unstamped and the round's end
+      // stamps all of them together, so no relaxation waits for a stamp
+      // exchange (a returning device atomic; nearly every wave has a lane
+      // with a far improvement).  An LDS overflow stamps at once.
+      auto far_defer = [&](uint32_t v) {
+        const uint32_t sl = atomicAdd(&s_nf, 1u);
+        if (sl < PQ_CAP) {
+          s_far[sl] = v;
+        } else if (atomicExch(&p.far_stamp[v], far_id) != far_id) {
+          st_agent(&far[fcur][nfar + atomicAdd(&C[1], 1u)], v);
+        }
+      };
       // the atomicMin of up to RELAX_GROUP edges issued together (the
       // updates of one row are independent), then their outcomes
       auto settle = [&](const uint32_t *v, const uint32_t *nd) {
@@ -415,8 +429,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
         for (int i = 0; i < RELAX_GROUP; i++) {
           if (nd[i] >= INFINIT || old[i] <= nd[i]) continue;
           if (nd[i] < Tk) owe(v[i], nd[i]);
-          else if (atomicExch(&p.far_stamp[v[i]], far_id) != far_id)
-            stage(v[i], s_far, &s_nf, &C[1], far[fcur], nfar);
+          else far_defer(v[i]);
         }
       };
       // relax u's out-edges with distance du (u's distance when it was
@@ -493,7 +506,25 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       __syncthreads();
       publish(s_near, s_nn, &C[0], q[cur ^ 1], 0, &s_base);
       __syncthreads();
-      publish(s_far, s_nf, &C[1], far[fcur], nfar, &s_base);
+      {
+        // the staged far vertices: every stamp exchange in flight together,
+        // then the ones this workgroup claimed first are published
+        constexpr uint32_t PER = (PQ_CAP + BLOCK - 1) / BLOCK;
+        const uint32_t n = s_nf < PQ_CAP ? s_nf : PQ_CAP;
+        uint32_t fv[PER], old[PER], keep = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < PER; r++) {
+          const uint32_t k = r * BLOCK + threadIdx.x;
+          fv[r] = k < n ? s_far[k] : 0u;
+          old[r] = k < n ? atomicExch(&p.far_stamp[fv[r]], far_id) : far_id;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < PER; r++) keep += old[r] != far_id;
+        uint32_t slot = block_reserve(&C[1], keep, s_scr);
+#pragma unroll
+        for (uint32_t r = 0; r < PER; r++)
+          if (old[r] != far_id) st_agent(&far[fcur][nfar + slot++], fv[r]);
+      }
       ok = grid_sync_atomic(&p.ctl->gs, gridDim.x, gen, p.timeout);
       step++;
       nn = ld_agent(&C[0]);

@@ -525,6 +525,33 @@ def _binary_tree(n, wmax, seed):
     return off, dst.astype(np.uint32), ww
 
 
+def test_sssp_persistent_far_list_overflow(ctx, oracle):
+    # unit weights and bucket width 1 on a binary tree of 2^20 - 1 vertices:
+    # every improvement lands in the far list, and the round that relaxes
+    # depth 18 (2^18 vertices spread over the workgroups) makes each
+    # workgroup stage several thousand far vertices, past its LDS list, so
+    # the overflow path (stamp and publish at once) runs beside the round-
+    # end stamping of the staged ones; a far vertex is listed once per epoch
+    off, col, w = _binary_tree((1 << 20) - 1, 1, 3)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=True) as g:
+        ref = oracle.sssp(host_csr(g), 0)
+        t0 = ctx.tuning
+        try:
+            for delta in (1, 2):
+                t = ctx.tuning
+                t.sssp_delta = delta
+                ctx.tuning = t
+                d, st = grus.sssp(g, 0)
+                print("delta %d rounds %d" % (delta, st.iterations), flush=True)
+                assert (d == ref).all()
+            # and from a leaf, where the far list fills from the bottom up
+            leaf = (1 << 20) - 2
+            d, _ = grus.sssp(g, leaf)
+            assert (d == oracle.sssp(host_csr(g), leaf)).all()
+        finally:
+            ctx.tuning = t0
+
+
 @pytest.mark.parametrize("wmax", [1, 40])
 def test_sssp_persistent_ring_spill(ctx, oracle, wmax):
     # a binary tree has degree <= 3 (the persistent path) and a frontier that
