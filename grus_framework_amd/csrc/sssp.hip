@@ -469,7 +469,11 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       };
       if (threadIdx.x == 0) s_nl = 0;
       __syncthreads();
-      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nn; i += stride) {
+      // the round's queue is dealt round-robin over the workgroups: a
+      // workgroup keeps relaxing what it improves, so with the queue in
+      // contiguous per-workgroup blocks a queue of a few thousand vertices
+      // would leave every chain of the round to the first few workgroups
+      for (uint64_t i = (uint64_t)threadIdx.x * gridDim.x + blockIdx.x; i < nn; i += stride) {
         const uint32_t u = ld_agent(&q[cur][i]);
         relax(u, ld_agent(&p.dist[u]));
       }
@@ -534,7 +538,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       // advance T to the bucket of the smallest live far distance
       const uint32_t Tk = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
       uint32_t m = 0xFFFFFFFFu;
-      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nfar; i += stride) {
+      for (uint64_t i = (uint64_t)threadIdx.x * gridDim.x + blockIdx.x; i < nfar; i += stride) {
         uint32_t d = ld_agent(&p.dist[ld_agent(&far[fcur][i])]);
         if (d >= Tk && d < m) m = d;
       }
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       clear_slot(p.ctl, step);
       if (threadIdx.x == 0) s_nn = s_nf = 0;
       __syncthreads();
-      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nfar; i += stride) {
+      for (uint64_t i = (uint64_t)threadIdx.x * gridDim.x + blockIdx.x; i < nfar; i += stride) {
         const uint32_t v = ld_agent(&far[fcur][i]);
         const uint32_t d = ld_agent(&p.dist[v]);
         if (d < Tk) continue;  // relaxed in an earlier bucket at its final distance
@@ -591,7 +595,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
 }  // namespace sssp
 
 namespace {
-uint32_t pick_delta(Graph &g, SsspWork &w) {
+uint32_t pick_delta(Graph &g, SsspWork &w, bool persistent) {
   Context &ctx = *g.ctx;
   if (ctx.tuning.sssp_delta) return ctx.tuning.sssp_delta;
   if (!g.weighted) return 0xFFFFFFFFu;  // unit weights: one bucket (BFS order)
@@ -611,7 +615,11 @@ uint32_t pick_delta(Graph &g, SsspWork &w) {
     w.mean_w = mean > 0 ? mean : 1.0;
   }
   double avg_deg = g.V ? (double)g.E / (double)g.V : 1.0;
-  double d = w.mean_w * std::max(2.0, 32.0 / std::max(avg_deg, 1.0));
+  // the persistent path relaxes a bucket within its round, so its buckets
+  // are wider (the 4096^2 grid, mean weight 500: widths 8000-16000 within
+  // 4% of the best, 4004 20% slower, 24000 21% slower)
+  const double span = persistent ? 80.0 : 32.0;
+  double d = w.mean_w * std::max(2.0, span / std::max(avg_deg, 1.0));
   return (uint32_t)std::min(std::max(d, 1.0), 4.0e9);
 }
 }  // namespace
@@ -677,9 +685,10 @@ void run_sssp_persistent(Graph &g, SsspWork &w, uint32_t src, uint64_t delta, ui
     const char *e = getenv("GRUS_PERSIST_BLOCKS");
     return e ? atoi(e) : 0;
   }();
-  // CUs / 4 measured best on the 4096^2 grid (32..512 workgroups: 142, 134,
-  // 138, 147, 222 ms)
-  unsigned want = env_blocks > 0 ? (unsigned)env_blocks : (unsigned)std::max(1, ctx.cu_count / 4);
+  // CUs / 2 measured best on the 4096^2 grid with the round's queue dealt
+  // round-robin (64 / 128 / 256 workgroups: 40.5 / 37.1 / 40.1 ms at their
+  // best bucket widths)
+  unsigned want = env_blocks > 0 ? (unsigned)env_blocks : (unsigned)std::max(1, ctx.cu_count / 2);
   unsigned grid = std::max(1u, std::min(want, (unsigned)(ctx.cu_count * per_cu)));
   void *args[] = {&p};
   GRUS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sssp::k_persistent),
@@ -708,8 +717,6 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   uint32_t *dist = g.result.as<uint32_t>();
   auto *cnt = reinterpret_cast<StepCounters *>(ctx.d_ctl.p);
   const StepCounters *hc = reinterpret_cast<const StepCounters *>(ctx.h_ctl);
-  const uint64_t delta = pick_delta(g, w);
-
   uint64_t src_deg;
   {
     uint64_t o[2];
@@ -723,6 +730,7 @@ void run_sssp(Graph &g, uint32_t src, grus_stats *st) {
   // runtime's residency check does not know the mask; nor beside other
   // contexts' kernels)
   const bool persistent = g.max_deg <= 32 && !no_persist && !ctx.cu_masked && !ctx.shared_device;
+  const uint64_t delta = pick_delta(g, w, persistent);
   // a round pulls when its frontier's out-edges exceed E / pull_div
   // (automatic: 16, measured best of 4..128 on RMAT s24; 0xFFFFFFFF: never)
   const uint32_t pull_div = ctx.tuning.sssp_pull_div ? ctx.tuning.sssp_pull_div : 16;
