@@ -104,7 +104,10 @@ typedef struct grus_tuning {
   double bfs_beta;        /* pull->push when n_f < V / beta      (default 64) */
   uint32_t bu_thread_edges; /* bottom-up: edges a lane scans before handing
                                the vertex to a wave (default 32) */
-  uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic) */
+  uint32_t sssp_delta;    /* SSSP bucket width (0 = automatic: mean weight x
+                             max(2, 32 / mean degree); on low-degree graphs,
+                             which run one persistent launch, x max(2, 80 /
+                             mean degree); unweighted graphs: one bucket) */
   uint32_t pr_layout;     /* PageRank edge layout: 0 automatic (= 4 where it
                              applies, else 2); 1 degree-sorted relabelled copy
                              (symmetric HBM graphs); 2 plain pull over the
