@@ -209,7 +209,8 @@ def main():
                          "(e.g. the values an unprofiled run's autotuner printed)")
     ap.add_argument("--only", default="",
                     help="comma list of extras to run: pagerank,cc,sssp (or sssp_rmat, "
-                         "sssp_grid),cu,serving,capped")
+                         "sssp_grid),cu,serving,capped; batched (the batched engines "
+                         "alone, run only when named)")
     ap.add_argument("--leg-limit", type=float, default=240.0,
                     help="seconds after which an extra leg with no result is abandoned "
                          "(0 = never)")
@@ -387,6 +388,40 @@ def run_extra(grus, O, ctx, g, h, threads, args, out, watch):
     if want("cu"):
         watch.leg("bfs_cu_scaling")
         out["bfs_cu_scaling"] = run_leg_child("cu", args)
+    if "batched" in only:  # named only: a kernel-trace target for the batched engines
+        watch.leg("batched_rmat_s22")
+        out["batched_rmat_s22"] = run_batched(grus)
+
+
+def run_batched(grus):
+    """The batched engines alone, without the server: three 128-query SSSP
+    batches and three 64-query BFS batches on the serving graph (RMAT s22,
+    w in [1,255], sources as in the serving leg), each set in its own timed
+    window, after one untimed batch of each."""
+    ctx = grus.Context(0)
+    g = grus.Graph.rmat(ctx, 22, weights=(1, 255), wseed=3)
+    off, _, _ = g.csr()
+    nz = np.nonzero(np.diff(off.astype(np.int64)))[0]
+    srcs = [int(x) for x in np.random.default_rng(5).choice(nz, 3 * 128 + 3 * 64)]
+    res = {}
+    grus.sssp_multi(g, srcs[:128], copy=False)
+    REGIONS.begin("batched_sssp128_rmat_s22_batch")
+    ms = [grus.sssp_multi(g, srcs[128 * i:128 * (i + 1)], copy=False)[1].device_ms
+          for i in range(3)]
+    REGIONS.end(3)
+    res["sssp_batch128_ms"] = round(float(np.median(ms)), 2)
+    b0 = 3 * 128
+    grus.bfs_multi(g, srcs[b0:b0 + 64], copy=False)
+    REGIONS.begin("batched_bfs64_rmat_s22_batch")
+    ms = [grus.bfs_multi(g, srcs[b0 + 64 * i:b0 + 64 * (i + 1)], copy=False)[1].device_ms
+          for i in range(3)]
+    REGIONS.end(3)
+    res["bfs_batch64_ms"] = round(float(np.median(ms)), 2)
+    log("batched s22: SSSP 128 queries %.1f ms, BFS 64 queries %.1f ms" %
+        (res["sssp_batch128_ms"], res["bfs_batch64_ms"]))
+    g.close()
+    ctx.close()
+    return res
 
 
 def run_pagerank(grus, O, g, h, threads, args, out):

@@ -1084,12 +1084,16 @@ void run_multi(Graph &g, MsKind kind, const uint32_t *srcs, int k, uint32_t *out
   GRUS_HIP(hipEventSynchronize(ctx.ev1));
   float ms = 0;
   GRUS_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+  // the reached counts of every slot in one pass over the engine's state;
+  // the rows are gathered one by one only when they are copied out
+  SlotMask all = 0;
+  for (int sl : slots) all |= slot_bit(sl);
+  uint64_t reached[128], checksum[128];
+  e.digest(all, reached, checksum);
   uint64_t vr = 0;
-  for (int i = 0; i < k; i++) {
-    uint64_t r = 0;
-    e.extract(slots[i], out ? out + (uint64_t)i * g.V : nullptr, &r, nullptr);
-    vr += r;
-  }
+  for (int sl : slots) vr += reached[sl];
+  if (out)
+    for (int i = 0; i < k; i++) e.extract(slots[i], out + (uint64_t)i * g.V, nullptr, nullptr);
   for (int sl : slots) e.release(sl);
   if (st) {
     *st = grus_stats{};

@@ -79,9 +79,24 @@ def test_sssp_multi_rmat(ctx, oracle, k):
     with grus.Graph.rmat(ctx, 12, weights=(1, 255), wseed=3) as g:
         h = host_csr(g)
         srcs = sources(h, max(k, 3), seed=3 + k)[:k]
-        out, _ = grus.sssp_multi(g, srcs)
+        out, st = grus.sssp_multi(g, srcs)
+        want = 0
         for i, s in enumerate(srcs):
-            assert (out[i] == oracle.sssp(h, s)).all(), (i, s)
+            ref = oracle.sssp(h, s)
+            assert (out[i] == ref).all(), (i, s)
+            want += int((ref != INF).sum())
+        # the reached count sums every slot's, copied out or not
+        assert st.vertices_reached == want
+        assert grus.sssp_multi(g, srcs, copy=False)[1].vertices_reached == want
+
+
+def test_bfs_multi_reached_count(ctx, oracle):
+    with grus.Graph.rmat(ctx, 12) as g:
+        h = host_csr(g)
+        srcs = sources(h, 40, seed=11)
+        want = sum(int((oracle.bfs(h, s) != INF).sum()) for s in srcs)
+        assert grus.bfs_multi(g, srcs)[1].vertices_reached == want
+        assert grus.bfs_multi(g, srcs, copy=False)[1].vertices_reached == want
 
 
 def test_sssp_multi_128_sources_and_errors(ctx, oracle):
