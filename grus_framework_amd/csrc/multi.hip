@@ -161,20 +161,25 @@ __global__ __launch_bounds__(BLOCK) void k_bfs_pull_long(CsrView in, const uint6
                                                          const uint64_t *front, uint64_t *next,
                                                          uint64_t running, const uint32_t *longq,
                                                          const MsCounters *cnt) {
-  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  // a workgroup per long row, its waves striding over the row (a hub's row
+  // would otherwise keep one wave busy long after the rest have finished);
+  // each wave stops once its own bits cover the lanes still looking, and
+  // ORs what it found into next[v], which only ever holds bits of need
+  const uint32_t wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
   const uint32_t n = cnt->long_count;
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    uint32_t v = longq[i];
-    uint64_t need = running & ~visit[v];
+  for (uint64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t v = longq[i];
+    const uint64_t need = running & ~visit[v];
     uint64_t acc = next[v];
-    uint64_t e1 = in.off[v + 1];
-    for (uint64_t base = in.off[v] + K_PULL; base < e1 && (acc & need) != need; base += WAVE) {
-      uint64_t e = base + lane_id();
-      uint64_t x = e < e1 ? front[edge_col(in, e)] : 0ull;
+    const uint64_t e1 = in.off[v + 1];
+    for (uint64_t base = in.off[v] + K_PULL + (uint64_t)wid * WAVE;
+         base < e1 && (acc & need) != need; base += (uint64_t)nw * WAVE) {
+      const uint64_t e = base + lane_id();
+      const uint64_t x = e < e1 ? front[edge_col(in, e)] : 0ull;
       acc |= wave_or(x);
     }
-    if (lane_id() == 0) next[v] = acc & need;
+    if (lane_id() == 0 && (acc & need))
+      atomicOr((unsigned long long *)&next[v], (unsigned long long)(acc & need));
   }
 }
 
