@@ -29,7 +29,7 @@ namespace mgg {
 // counters of one frontier step, in the context's device control block
 struct StepCounters {
   uint32_t next_count;   // vertices appended to the next queue
-  uint32_t long_count;   // bottom-up vertices handed to wave-cooperative scan
+  uint32_t long_count;   // bottom-up vertices handed to a cooperative scan
   uint64_t next_edges;   // sum of out-degrees of the next frontier
   uint32_t far_count;    // vertices appended to the far queue (SSSP)
   uint32_t min_far;      // minimum live far distance (SSSP threshold advance)

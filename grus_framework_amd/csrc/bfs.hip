@@ -18,8 +18,8 @@
 //     label==INFINIT, read from the visited bitmap) and scans its in-edges
 //     for a parent in the current frontier bitmap, stopping at the first
 //     hit (the reference scans every in-edge, kernel.cuh:31-37).  After
-//     `bu_thread_edges` edges a vertex is handed to a wave-cooperative scan
-//     (64 edges per step, ballot early exit).  The wave ballots its found
+//     `bu_thread_edges` edges a vertex is handed to a workgroup-cooperative
+//     scan (its waves 64 edges per step each, ballot early exit).  The wave ballots its found
 //     lanes and writes the next frontier word and visited word with plain
 //     stores -- no atomics -- and its level stores are coalesced.
 //   direction switch (Beamer et al., SC'12): push->pull when the
@@ -481,31 +481,37 @@ __global__ __launch_bounds__(BLOCK) void k_pull_sparse(
                    (unsigned long long *)&cnt->next_edges);
 }
 
-// wave-cooperative scan of the vertices whose first K in-edges held no parent
+// workgroup-cooperative scan of the vertices whose first K in-edges held no parent
 __global__ __launch_bounds__(BLOCK) void k_pull_long(CsrView in, const uint64_t *out_off,
                                                      const uint64_t *front, uint64_t *visited,
                                                      uint64_t *front_next, uint32_t *label,
                                                      uint32_t next_level, uint32_t K,
                                                      const uint32_t *longq, StepCounters *cnt) {
-  uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
-  uint32_t lane = lane_id();
+  // a workgroup per long row, its waves striding over the row and stopping
+  // as soon as one of them sees a parent (an LDS flag): a row whose parent
+  // lies deep in it no longer holds one wave while the rest have finished
+  __shared__ uint32_t s_found;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
   const uint32_t n = cnt->long_count;
   uint32_t my_found = 0;
   uint64_t my_edges = 0;
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    uint32_t v = longq[i];
-    uint64_t end = in.off[v + 1];
-    bool found = false;
-    for (uint64_t base = in.off[v] + K; base < end; base += WAVE) {
-      uint64_t e = base + lane;
-      bool hit = e < end && bm_test(front, edge_col(in, e));
+  for (uint64_t i = blockIdx.x; i < n; i += gridDim.x) {  // block-uniform
+    if (threadIdx.x == 0) s_found = 0;
+    __syncthreads();
+    const uint32_t v = longq[i];
+    const uint64_t end = in.off[v + 1];
+    for (uint64_t base = in.off[v] + K + (uint64_t)wid * WAVE; base < end;
+         base += (uint64_t)nw * WAVE) {
+      if (__hip_atomic_load(&s_found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+      const uint64_t e = base + lane;
+      const bool hit = e < end && bm_test(front, edge_col(in, e));
       if (__ballot(hit)) {
-        found = true;
+        if (lane == 0) __hip_atomic_store(&s_found, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         break;
       }
     }
-    if (found && lane == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0 && s_found) {
       label[v] = next_level;
       unsigned long long bit = 1ull << (v & 63);
       atomicOr((unsigned long long *)&front_next[v >> 6], bit);
@@ -513,8 +519,9 @@ __global__ __launch_bounds__(BLOCK) void k_pull_long(CsrView in, const uint64_t 
       my_found++;
       my_edges += out_off[v + 1] - out_off[v];
     }
+    __syncthreads();  // s_found read before the next row resets it
   }
-  if (lane == 0 && my_found) {
+  if (threadIdx.x == 0 && my_found) {
     atomicAdd(&cnt->next_count, my_found);
     atomicAdd((unsigned long long *)&cnt->next_edges, my_edges);
   }
