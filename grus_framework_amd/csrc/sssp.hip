@@ -372,6 +372,10 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
   int cur = 0, fcur = 0;
   bool ok = true;
   uint32_t passes = 0;
+  // the smallest distance in the far list, known after a split that found
+  // its bucket empty (until the next round adds far vertices)
+  uint32_t fmin = 0xFFFFFFFFu;
+  bool fmin_known = false;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   while (ok) {
     uint32_t *C = p.ctl->ring[step & 3];
@@ -533,51 +537,52 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(PParams p) {
       step++;
       nn = ld_agent(&C[0]);
       nfar += ld_agent(&C[1]);
+      fmin_known = false;  // the round may have added far vertices below it
       cur ^= 1;
     } else if (nfar > 0) {
-      // advance T to the bucket of the smallest live far distance
+      // Advance T by one bucket and split the far list in the same pass.
+      // T is a multiple of delta and a far vertex lies at T or beyond, so
+      // when the bucket [T, T + delta) holds a far vertex this is the bucket
+      // of the smallest far distance.  The split also takes the minimum of
+      // what stays far: when the bucket came out empty, the next split goes
+      // straight to the bucket of that minimum (valid until a round adds
+      // far vertices again).
       const uint32_t Tk = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
-      uint32_t m = 0xFFFFFFFFu;
-      for (uint64_t i = (uint64_t)threadIdx.x * gridDim.x + blockIdx.x; i < nfar; i += stride) {
-        uint32_t d = ld_agent(&p.dist[ld_agent(&far[fcur][i])]);
-        if (d >= Tk && d < m) m = d;
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        uint32_t x = __shfl_xor(m, o);
-        m = x < m ? x : m;
-      }
-      if (lane_id() == 0 && m != 0xFFFFFFFFu) atomicMin(&C[2], m);
-      ok = grid_sync_atomic(&p.ctl->gs, gridDim.x, gen, p.timeout);
-      step++;
-      const uint32_t mn = ld_agent(&C[2]);
-      if (!ok || mn == 0xFFFFFFFFu) break;  // every far entry was stale
-      const uint64_t Tn = ((uint64_t)mn / p.delta + 1) * p.delta;
+      const uint64_t Tn = fmin_known ? ((uint64_t)fmin / p.delta + 1) * p.delta : T + p.delta;
       const uint32_t Tnk = Tn < 0xFFFFFFFFull ? (uint32_t)Tn : 0xFFFFFFFFu;
       far_id = ++stamp;
       const uint32_t near_id = ++stamp;
-      uint32_t *C2 = p.ctl->ring[step & 3];
-      clear_slot(p.ctl, step);
       if (threadIdx.x == 0) s_nn = s_nf = 0;
       __syncthreads();
+      uint32_t m = 0xFFFFFFFFu;
       for (uint64_t i = (uint64_t)threadIdx.x * gridDim.x + blockIdx.x; i < nfar; i += stride) {
         const uint32_t v = ld_agent(&far[fcur][i]);
         const uint32_t d = ld_agent(&p.dist[v]);
         if (d < Tk) continue;  // relaxed in an earlier bucket at its final distance
         if (d < Tnk) {
           if (atomicExch(&p.near_stamp[v], near_id) != near_id)
-            stage(v, s_near, &s_nn, &C2[0], q[cur], 0);
-        } else if (atomicExch(&p.far_stamp[v], far_id) != far_id) {
-          stage(v, s_far, &s_nf, &C2[1], far[fcur ^ 1], 0);
+            stage(v, s_near, &s_nn, &C[0], q[cur], 0);
+        } else {
+          m = d < m ? d : m;
+          if (atomicExch(&p.far_stamp[v], far_id) != far_id)
+            stage(v, s_far, &s_nf, &C[1], far[fcur ^ 1], 0);
         }
       }
+      for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(m, o);
+        m = x < m ? x : m;
+      }
+      if (lane_id() == 0 && m != 0xFFFFFFFFu) atomicMin(&C[2], m);
       __syncthreads();
-      publish(s_near, s_nn, &C2[0], q[cur], 0, &s_base);
+      publish(s_near, s_nn, &C[0], q[cur], 0, &s_base);
       __syncthreads();
-      publish(s_far, s_nf, &C2[1], far[fcur ^ 1], 0, &s_base);
+      publish(s_far, s_nf, &C[1], far[fcur ^ 1], 0, &s_base);
       ok = grid_sync_atomic(&p.ctl->gs, gridDim.x, gen, p.timeout);
       step++;
-      nn = ld_agent(&C2[0]);
-      nfar = ld_agent(&C2[1]);
+      nn = ld_agent(&C[0]);
+      nfar = ld_agent(&C[1]);
+      fmin = ld_agent(&C[2]);
+      fmin_known = nn == 0;
       fcur ^= 1;
       T = Tn;
     } else {
