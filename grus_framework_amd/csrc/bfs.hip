@@ -177,18 +177,30 @@ __global__ __launch_bounds__(BLOCK) void k_stamp_commit(const uint8_t *stamp, ui
 
 __global__ void k_init(uint32_t *label, uint64_t V, uint64_t *visited, const uint64_t *unreachable,
                        uint64_t W, uint32_t src, uint32_t *q0) {
-  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  uint64_t n = V > W ? V : W;
-  for (; i < n; i += stride) {
-    if (i < V) label[i] = (i == src) ? 0u : INFINIT;  // BFSInit bfs.cu:21
-    if (i < W) {
-      uint64_t m = unreachable ? unreachable[i] : 0ull;
-      if (i == (src >> 6)) m |= 1ull << (src & 63);
-      visited[i] = m;
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // levels (BFSInit bfs.cu:21): INFINIT, 0 at the source, four per 16-byte
+  // store (the array is 256-byte aligned), the last V % 4 one by one
+  const uint64_t V4 = V / 4;
+  uint4 *l4 = reinterpret_cast<uint4 *>(label);
+  for (uint64_t i = t; i < V4; i += stride) {
+    uint4 x = make_uint4(INFINIT, INFINIT, INFINIT, INFINIT);
+    if (i == (uint64_t)(src >> 2)) {
+      const uint32_t c = src & 3;
+      x.x = c == 0 ? 0u : x.x;
+      x.y = c == 1 ? 0u : x.y;
+      x.z = c == 2 ? 0u : x.z;
+      x.w = c == 3 ? 0u : x.w;
     }
-    if (i == 0) q0[0] = src;
+    l4[i] = x;
   }
+  for (uint64_t i = V4 * 4 + t; i < V; i += stride) label[i] = i == src ? 0u : INFINIT;
+  for (uint64_t i = t; i < W; i += stride) {
+    uint64_t m = unreachable ? unreachable[i] : 0ull;
+    if (i == (src >> 6)) m |= 1ull << (src & 63);
+    visited[i] = m;
+  }
+  if (t == 0) q0[0] = src;
 }
 
 // first[v] = lowest-id in-neighbour of v (rows are sorted), GRUS_NO_VERTEX
