@@ -219,7 +219,9 @@ def main():
     ap.add_argument("--regions-out", default="",
                     help="write the host-clock windows of the timed loops to this JSON file")
     ap.add_argument("--leg-child", default="", choices=["", "serving", "cu", "capped"],
-                    help=argparse.SUPPRESS)  # one leg, run as a child process
+                    help="run one extra leg in this process and print its result as JSON "
+                         "(what the full run starts as a child process; also the target "
+                         "for profiling a leg)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
