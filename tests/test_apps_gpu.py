@@ -552,6 +552,30 @@ def test_sssp_persistent_far_list_overflow(ctx, oracle):
             ctx.tuning = t0
 
 
+def test_sssp_persistent_empty_buckets(ctx, oracle):
+    # weights up to 100,000 on a binary tree (the persistent path) with
+    # buckets much narrower than most weights: most far splits find the next
+    # bucket [T, T + delta) empty and are followed by a split at the bucket
+    # of the smallest far distance, which the empty split measured
+    off, col, w = _binary_tree((1 << 12) - 1, 100000, 5)
+    with grus.Graph.from_csr(ctx, off, col, w, symmetric=True) as g:
+        h = host_csr(g)
+        ref = oracle.sssp(h, 0)
+        t0 = ctx.tuning
+        try:
+            for delta in (7, 1000, 50000):
+                t = ctx.tuning
+                t.sssp_delta = delta
+                ctx.tuning = t
+                d, st = grus.sssp(g, 0)
+                print("delta %d rounds %d" % (delta, st.iterations), flush=True)
+                assert (d == ref).all(), delta
+            leaf = (1 << 12) - 2
+            assert (grus.sssp(g, leaf)[0] == oracle.sssp(h, leaf)).all()
+        finally:
+            ctx.tuning = t0
+
+
 @pytest.mark.parametrize("wmax", [1, 40])
 def test_sssp_persistent_ring_spill(ctx, oracle, wmax):
     # a binary tree has degree <= 3 (the persistent path) and a frontier that
